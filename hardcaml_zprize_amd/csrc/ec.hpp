@@ -1,0 +1,147 @@
+// BLS12-377 G1 point arithmetic on gfx950 (and on the host for the final
+// conversion), in the twisted-Edwards model of the reference
+// (a = -1, d, k = 2d; zprize/msm_pippenger/host/msm_compute/include/bls12_377_g1/bls12_377_g1.h:31-54,
+// libs/twisted_edwards/model/twisted_edwards_curve.ml).
+//
+// Accumulators are extended coordinates (X : Y : Z : T), x = X/Z, y = Y/Z,
+// T = XY/Z, each coordinate a field value < 2q.  Base points are stored as
+// the precomputed affine triple (y - x, y + x, 2d x y), canonical, packed
+// (144 bytes): the mixed addition then needs 7 multiplications (hwcd-3 with
+// Z2 = 1, the operation of twisted_edwards_curve.ml:145-170 without the
+// reference's 1/2 and 4 scalings), and subtracting a point swaps the first
+// two entries and negates the third (twisted_edwards_curve.ml:151-152).
+#pragma once
+#include "fq377.hpp"
+
+namespace zp {
+
+struct Xyzt {
+  F x, y, z, t;
+};
+
+struct PrePt {
+  F ymx, ypx, t2d;
+};
+
+struct PackedPre {
+  Packed ymx, ypx, t2d;
+};
+
+ZP_HD Xyzt ec_identity(const FConsts &K) {
+  Xyzt r;
+  r.x = f_zero();
+  r.y = K.one;
+  r.z = K.one;
+  r.t = f_zero();
+  return r;
+}
+
+// acc += (neg ? -P : P), P in precomputed form.  7 multiplications.
+ZP_HD void ec_madd(Xyzt &a, const PrePt &p, bool neg, const FConsts &K) {
+  const F &p0 = neg ? p.ypx : p.ymx;
+  const F &p1 = neg ? p.ymx : p.ypx;
+  const F A = f_mul(f_sub_kq(a.y, a.x, K.q2), p0);  // (Y - X)(y2 - x2) ; Y-X < 4q
+  const F B = f_mul(f_add(a.y, a.x), p1);           // (Y + X)(y2 + x2) ; < 4q
+  const F C = f_mul(a.t, p.t2d);                    // T 2d t2
+  const F D = f_add(a.z, a.z);                      // 2 Z ; < 4q
+  const F E = f_sub_kq(B, A, K.q2);                 // < 4q
+  const F H = f_add(B, A);                          // < 4q
+  // -P negates t2: F and G swap.
+  const F Fm = neg ? f_add(D, C) : f_sub_kq(D, C, K.q2);  // < 6q
+  const F Gm = neg ? f_sub_kq(D, C, K.q2) : f_add(D, C);  // < 6q
+  a.x = f_mul(E, Fm);
+  a.y = f_mul(Gm, H);
+  a.t = f_mul(E, H);
+  a.z = f_mul(Fm, Gm);
+}
+
+// a += b, both extended (hwcd-3 unified addition, bls12_377_g1.h:449-491).
+// 9 multiplications.  Complete for the points this code adds.
+ZP_HD void ec_add(Xyzt &a, const Xyzt &b, const FConsts &K) {
+  const F A = f_mul(f_sub_kq(a.y, a.x, K.q2), f_sub_kq(b.y, b.x, K.q2));
+  const F B = f_mul(f_add(a.y, a.x), f_add(b.y, b.x));
+  const F C = f_mul(f_mul(a.t, b.t), K.k2d);
+  const F zz = f_mul(a.z, b.z);
+  const F D = f_add(zz, zz);
+  const F E = f_sub_kq(B, A, K.q2);
+  const F H = f_add(B, A);
+  const F Fm = f_sub_kq(D, C, K.q2);
+  const F Gm = f_add(D, C);
+  a.x = f_mul(E, Fm);
+  a.y = f_mul(Gm, H);
+  a.t = f_mul(E, H);
+  a.z = f_mul(Fm, Gm);
+}
+
+// a = 2a (dbl-2008-hwcd with a = -1).  8 multiplications.
+ZP_HD void ec_dbl(Xyzt &a, const FConsts &K) {
+  const F A = f_sqr(a.x);
+  const F B = f_sqr(a.y);
+  const F zz = f_sqr(a.z);
+  const F C = f_add(zz, zz);                          // < 4q
+  const F S = f_sqr(f_add(a.x, a.y));                 // (X+Y)^2
+  const F AB = f_add(A, B);                           // < 4q
+  const F E = f_sub_kq(S, AB, K.q4);                  // < 6q
+  const F G = f_sub_kq(B, A, K.q2);                   // D + B = B - A ; < 4q
+  const F Fm = f_sub_kq(G, C, K.q4);                  // < 8q
+  const F H = f_sub_kq(f_zero(), AB, K.q4);           // D - B = -(A + B) ; < 4q
+  a.x = f_mul(E, Fm);
+  a.y = f_mul(G, H);
+  a.t = f_mul(E, H);
+  a.z = f_mul(Fm, G);
+}
+
+ZP_HD PrePt pre_unpack(const PackedPre &p) {
+  PrePt r;
+  r.ymx = f_unpack(p.ymx);
+  r.ypx = f_unpack(p.ypx);
+  r.t2d = f_unpack(p.t2d);
+  return r;
+}
+
+ZP_HD PackedPre pre_pack(const PrePt &p) {
+  PackedPre r;
+  r.ymx = f_pack(f_canonical(p.ymx));
+  r.ypx = f_pack(f_canonical(p.ypx));
+  r.t2d = f_pack(f_canonical(p.t2d));
+  return r;
+}
+
+// Affine twisted-Edwards (x, y) to the precomputed triple.
+ZP_HD PrePt pre_from_affine(const F &x, const F &y, const FConsts &K) {
+  PrePt r;
+  r.ymx = f_sub_kq(y, x, K.q2);
+  r.ypx = f_add(y, x);
+  r.t2d = f_mul(f_mul(x, y), K.d2);
+  return r;
+}
+
+// Field halving: a / 2 for normalized a < 8q (adds q when odd, then shifts).
+ZP_HD F f_half(const F &a) {
+  F t = a;
+  if (t.l[0] & 1) {
+#pragma unroll
+    for (int i = 0; i < 14; i++) t.l[i] += q28(i);
+    f_carry(t);
+  }
+  F r;
+#pragma unroll
+  for (int i = 0; i < 13; i++) r.l[i] = (t.l[i] >> 1) | ((t.l[i + 1] & 1) << 27);
+  r.l[13] = t.l[13] >> 1;
+  return r;
+}
+
+// Precomputed triple back to extended coordinates (x = (p1 - p0)/2, y = (p1 + p0)/2, Z = 1).
+ZP_HD Xyzt ext_from_pre(const PrePt &p, bool neg, const FConsts &K) {
+  Xyzt r;
+  F x = f_half(f_sub_kq(p.ypx, p.ymx, K.q2));
+  const F y = f_half(f_add(p.ypx, p.ymx));
+  if (neg) x = f_sub_kq(f_zero(), x, K.q4);
+  r.x = x;
+  r.y = y;
+  r.z = K.one;
+  r.t = f_mul(x, y);
+  return r;
+}
+
+}  // namespace zp

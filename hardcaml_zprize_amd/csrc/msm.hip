@@ -1,0 +1,879 @@
+// BLS12-377 G1 MSM: context, point preprocessing, the per-batch pipeline and
+// the MSM half of the C ABI (msm_init / msm_mult of
+// zprize/msm_pippenger/host/driver/driver.cpp:603-636).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "msm_host.hpp"
+#include "msm_kernels.hpp"
+
+namespace zp {
+
+static std::once_flag g_consts_once;
+static FConsts g_K;
+
+static const FConsts &consts() {
+  std::call_once(g_consts_once, [] { g_K = make_fconsts(); });
+  return g_K;
+}
+
+// ------------------------------------------------------------ layouts --
+// The reference's 20 windows: 13 of 13 bits then 7 of 12 bits, top window
+// unsigned (zprize/msm_pippenger/hardcaml/src/config_utils.ml:7-41).
+static MsmLayout layout_reference() {
+  MsmLayout L{};
+  L.W = 20;
+  int off = 0;
+  for (int w = 0; w < 20; w++) {
+    L.width[w] = w < 13 ? 13 : 12;
+    L.off[w] = off;
+    off += L.width[w];
+  }
+  return L;
+}
+
+// Uniform windows of c bits (top window takes what is left of the 253 bits).
+static MsmLayout layout_uniform(int c) {
+  MsmLayout L{};
+  L.W = (253 + c - 1) / c;
+  for (int w = 0; w < L.W; w++) {
+    L.off[w] = w * c;
+    L.width[w] = std::min(c, 253 - w * c);
+  }
+  return L;
+}
+
+static void layout_finish(MsmLayout &L) {
+  uint32_t base = 0;
+  for (int w = 0; w < L.W; w++) {
+    if (w == L.W - 1)
+      L.nb[w] = 1u << (253 - L.off[w]);  // unsigned top window: slice + carry
+    else
+      L.nb[w] = 1u << (L.width[w] - 1);
+    L.base[w] = base;
+    base += L.nb[w];
+  }
+  L.K = base;
+}
+
+// Window width for n points (the size-class policy; "window=<c>" overrides).
+static int auto_window(size_t n) {
+  int lg = 0;
+  while ((1ull << (lg + 1)) <= n) lg++;
+  if (lg <= 8) return 8;
+  if (lg >= 24) return 16;
+  return std::max(8, lg - 6);
+}
+
+struct MsmOptions {
+  int window = 0;      // 0 = auto
+  bool reference = false;
+  int log_u = -1;      // entries per work unit, -1 = auto
+  int log_g = -1;      // buckets per window group, -1 = auto
+};
+
+static MsmOptions parse_options(const char *s, size_t len) {
+  MsmOptions o;
+  std::string str = (s && len) ? std::string(s, len) : std::string();
+  size_t pos = 0;
+  while (pos < str.size()) {
+    size_t end = str.find_first_of(", ;", pos);
+    if (end == std::string::npos) end = str.size();
+    std::string kv = str.substr(pos, end - pos);
+    size_t eq = kv.find('=');
+    if (eq != std::string::npos) {
+      std::string k = kv.substr(0, eq), v = kv.substr(eq + 1);
+      if (k == "window") {
+        if (v == "reference")
+          o.reference = true;
+        else
+          o.window = atoi(v.c_str());
+      } else if (k == "log_u") {
+        o.log_u = atoi(v.c_str());
+      } else if (k == "log_g") {
+        o.log_g = atoi(v.c_str());
+      }
+    }
+    pos = end + 1;
+  }
+  return o;
+}
+
+static MsmLayout choose_layout(const MsmOptions &o, size_t n) {
+  MsmLayout L = o.reference ? layout_reference() : layout_uniform(o.window ? o.window : auto_window(n));
+  layout_finish(L);
+  return L;
+}
+
+// ----------------------------------------------------------- kernels --
+__global__ void k_pre_a(const g1_affine_t *in, uint64_t n, FConsts K, F *u_out, F *v_out, F *den_out,
+                        uint8_t *flag) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const g1_affine_t &p = in[i];
+    if (p.infinity) {  // bls12_377_g1.h:647-651: the twisted-Edwards identity
+      flag[i] = 2;
+      den_out[i] = K.one;
+      continue;
+    }
+    Packed px, py;
+    const uint32_t *wx = reinterpret_cast<const uint32_t *>(p.x.data);
+    const uint32_t *wy = reinterpret_cast<const uint32_t *>(p.y.data);
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      px.w[j] = wx[j];
+      py.w[j] = wy[j];
+    }
+    const F x = f_mul(f_unpack(px), K.to_int);
+    const F y = f_mul(f_unpack(py), K.to_int);
+    // affineWeierstrassToMontgomery (bls12_377_g1.h:361-365): u = s (x - alpha), v = s y.
+    const F u = f_mul(f_sub_kq(x, K.alpha, K.q2), K.mont_s);
+    const F v = f_mul(K.mont_s, y);
+    const F up1 = f_add(u, K.one);
+    const F den = f_mul(v, up1);
+    u_out[i] = u;
+    v_out[i] = v;
+    // affineMontgomeryToTwistedEdwards fails iff v = 0 or u = -1 (:377-388).
+    if (f_is_zero_canon(den)) {
+      flag[i] = 0;
+      den_out[i] = K.one;
+    } else {
+      flag[i] = 1;
+      den_out[i] = den;
+    }
+  }
+}
+
+// In-place inversion of a[0..n) in chunks of `chunk` per thread (Montgomery's
+// trick; every element must be non-zero).  tmp holds the prefix products.
+__global__ void k_batch_inv(F *a, uint64_t n, uint32_t chunk, F *tmp, FConsts K) {
+  const uint64_t nchunks = (n + chunk - 1) / chunk;
+  for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nchunks;
+       c += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t b0 = c * chunk, b1 = min(n, b0 + chunk);
+    F acc = K.one;
+    for (uint64_t i = b0; i < b1; i++) {
+      acc = f_mul(acc, a[i]);
+      tmp[i] = acc;
+    }
+    F inv = f_inv(acc, K.one);
+    for (uint64_t i = b1; i-- > b0;) {
+      const F prev = (i > b0) ? tmp[i - 1] : K.one;
+      const F ai = a[i];
+      a[i] = f_mul(inv, prev);
+      inv = f_mul(inv, ai);
+    }
+  }
+}
+
+__global__ void k_pre_c(const F *u_in, const F *v_in, const F *inv_den, const uint8_t *flag, uint64_t n, FConsts K,
+                        PackedPre *out, uint8_t *conv) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint8_t f = flag[i];
+    PrePt pre;
+    if (f != 1) {
+      // identity (y - x, y + x, 2dxy) = (1, 1, 0); unused when not convertible
+      pre.ymx = K.one;
+      pre.ypx = K.one;
+      pre.t2d = f_zero();
+    } else {
+      const F u = u_in[i], v = v_in[i], id = inv_den[i];
+      const F up1 = f_add(u, K.one);
+      const F inv_v = f_mul(up1, id);
+      const F inv_up1 = f_mul(v, id);
+      const F x = f_mul(f_mul(K.ts, u), inv_v);              // twisted_scale u / v
+      const F y = f_mul(f_sub_kq(u, K.one, K.q2), inv_up1);  // (u - 1) / (u + 1)
+      pre = pre_from_affine(x, y, K);
+    }
+    out[i] = pre_pack(pre);
+    conv[i] = f != 0;
+  }
+}
+
+// Fixed-base comb: acc = sum_j T[j][byte_j(k)] with T[j][m] = m 2^(8j) G.
+__global__ void k_gen_a(const biginteger256_t *ks, uint64_t n, const PackedPre *table, FConsts K, Xyzt *acc_out,
+                        F *den_out, uint8_t *flag) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    Xyzt acc = ec_identity(K);
+    for (int j = 0; j < 32; j++) {
+      const uint32_t m = (uint32_t)(ks[i].data[j >> 3] >> (8 * (j & 7))) & 0xff;
+      if (m) ec_madd(acc, pre_unpack(table[j * 256 + m]), false, K);
+    }
+    acc_out[i] = acc;
+    // twisted Edwards -> Montgomery needs 1 / ((Z - Y) X) (bls12_377_g1.h:389-402).
+    const F den = f_mul(f_sub_kq(acc.z, acc.y, K.q2), acc.x);
+    if (f_is_zero_canon(den)) {
+      flag[i] = 0;
+      den_out[i] = K.one;
+    } else {
+      flag[i] = 1;
+      den_out[i] = den;
+    }
+  }
+}
+
+// Extended twisted Edwards + 1/((Z-Y)X) -> affine Weierstrass in the wire form.
+__device__ __forceinline__ void te_to_wire(const Xyzt &a, const F &inv, const FConsts &K, biginteger384_t &xo,
+                                           biginteger384_t &yo) {
+  const F zpy = f_add(a.z, a.y);
+  const F u = f_mul(f_mul(zpy, a.x), inv);                  // (1 + y) / (1 - y)
+  const F v = f_mul(f_mul(f_mul(zpy, a.z), inv), K.ts);     // twisted_scale u / x
+  const F xw = f_add(f_mul(u, K.inv_cb), K.ca_3cb);         // bls12_377_g1.h:366-374
+  const F yw = f_mul(v, K.inv_cb);
+  const Packed px = f_pack(f_canonical(f_mul(xw, K.to_wire)));
+  const Packed py = f_pack(f_canonical(f_mul(yw, K.to_wire)));
+  uint32_t *dx = reinterpret_cast<uint32_t *>(xo.data);
+  uint32_t *dy = reinterpret_cast<uint32_t *>(yo.data);
+#pragma unroll
+  for (int j = 0; j < 12; j++) {
+    dx[j] = px.w[j];
+    dy[j] = py.w[j];
+  }
+}
+
+__global__ void k_gen_c(const Xyzt *acc, const F *inv, const uint8_t *flag, uint64_t n, FConsts K, g1_affine_t *out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    g1_affine_t r;
+    if (!flag[i]) {
+#pragma unroll
+      for (int j = 0; j < 6; j++) r.x.data[j] = r.y.data[j] = 0;
+      r.infinity = true;
+    } else {
+      te_to_wire(acc[i], inv[i], K, r.x, r.y);
+      r.infinity = false;
+    }
+    out[i] = r;
+  }
+}
+
+// Diagnostics: wire-form Montgomery product through the device multiply.
+__global__ void k_debug_mul(const uint64_t *a, const uint64_t *b, uint64_t *o, uint64_t n, FConsts K) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    Packed pa, pb;
+    for (int j = 0; j < 6; j++) {
+      pa.w[2 * j] = (uint32_t)a[6 * i + j];
+      pa.w[2 * j + 1] = (uint32_t)(a[6 * i + j] >> 32);
+      pb.w[2 * j] = (uint32_t)b[6 * i + j];
+      pb.w[2 * j + 1] = (uint32_t)(b[6 * i + j] >> 32);
+    }
+    const F x = f_mul(f_unpack(pa), K.to_int), y = f_mul(f_unpack(pb), K.to_int);
+    const Packed r = f_pack(f_canonical(f_mul(f_mul(x, y), K.to_wire)));
+    for (int j = 0; j < 6; j++) o[6 * i + j] = (uint64_t)r.w[2 * j] | ((uint64_t)r.w[2 * j + 1] << 32);
+  }
+}
+
+__global__ void k_debug_pre_wire(const PackedPre *pts, uint64_t i0, uint64_t n, FConsts K, uint64_t *o) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const PrePt p = pre_unpack(pts[i0 + i]);
+    const F v[3] = {p.ymx, p.ypx, p.t2d};
+    for (int c = 0; c < 3; c++) {
+      const Packed r = f_pack(f_canonical(f_mul(v[c], K.to_wire)));
+      for (int j = 0; j < 6; j++) o[18 * i + 6 * c + j] = (uint64_t)r.w[2 * j] | ((uint64_t)r.w[2 * j + 1] << 32);
+    }
+  }
+}
+
+// Fill kernel (used instead of hipMemsetAsync so the sequence can be graph-captured).
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static unsigned grid_for(uint64_t n, unsigned threads = MSM_THREADS, unsigned cap = 65535) {
+  uint64_t g = (n + threads - 1) / threads;
+  if (g == 0) g = 1;
+  return (unsigned)std::min<uint64_t>(g, cap);
+}
+
+// --------------------------------------------------------- workspace --
+struct Workspace {
+  size_t cap_K = 0, cap_entries = 0, cap_units = 0, cap_groups = 0, cap_scalars = 0;
+  uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
+  uint32_t *ubucket = nullptr, *vals = nullptr;
+  Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
+  uint64_t *scalars = nullptr;
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+
+  void release() {
+    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, partial, buckets, groups, winsum, scalars, scan_tmp};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    *this = Workspace();
+  }
+};
+
+template <typename T>
+static int grow(T *&p, size_t &cap, size_t need) {
+  if (need <= cap && p) return ZP_OK;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(need, 1) * sizeof(T)) != hipSuccess) {
+    cap = 0;
+    set_last_error("MSM workspace: allocation of %zu bytes failed", need * sizeof(T));
+    return ZP_ERR_OUT_OF_MEMORY;
+  }
+  cap = need;
+  return ZP_OK;
+}
+
+static int ensure_workspace(Workspace &ws, const MsmLayout &L, uint32_t log_u, uint32_t log_g, size_t n,
+                            size_t n_scalars, hipStream_t stream) {
+  const size_t K = L.K;
+  size_t capK = ws.cap_K;
+  if (K + 1 > ws.cap_K) {
+    size_t c;
+    c = 0;
+    ZP_TRY(grow(ws.cnt, c, K + 1));
+    c = 0;
+    ZP_TRY(grow(ws.off, c, K + 1));
+    c = 0;
+    ZP_TRY(grow(ws.cursor, c, K + 1));
+    c = 0;
+    ZP_TRY(grow(ws.nunits, c, K + 1));
+    c = 0;
+    ZP_TRY(grow(ws.ustart, c, K + 1));
+    c = 0;
+    ZP_TRY(grow(ws.buckets, c, K + 1));
+    capK = K + 1;
+    // scan temp storage for K + 1 elements
+    size_t bytes = 0;
+    ZP_HIP(rocprim::exclusive_scan(nullptr, bytes, ws.cnt, ws.off, 0u, K + 1, rocprim::plus<uint32_t>(), stream));
+    if (ws.scan_tmp) (void)hipFree(ws.scan_tmp);
+    ws.scan_tmp = nullptr;
+    ZP_HIP(hipMalloc(&ws.scan_tmp, std::max<size_t>(bytes, 16)));
+    ws.scan_tmp_bytes = bytes;
+  }
+  ws.cap_K = capK;
+  const size_t entries = n * (size_t)L.W;
+  ZP_TRY(grow(ws.vals, ws.cap_entries, entries));
+  const size_t units = (entries >> log_u) + K + 1;
+  {
+    size_t c = ws.cap_units;
+    ZP_TRY(grow(ws.ubucket, c, units));
+    size_t c2 = ws.cap_units;
+    ZP_TRY(grow(ws.partial, c2, units));
+    ws.cap_units = std::max(c, c2);
+  }
+  size_t groups = 0;
+  for (int w = 0; w < L.W; w++) groups += (L.nb[w] + (1u << log_g) - 1) >> log_g;
+  ZP_TRY(grow(ws.groups, ws.cap_groups, groups + 1));
+  if (!ws.winsum) ZP_HIP(hipMalloc(&ws.winsum, MSM_MAXW * sizeof(Xyzt)));
+  if (n_scalars) ZP_TRY(grow(ws.scalars, ws.cap_scalars, 4 * n_scalars));
+  return ZP_OK;
+}
+
+static uint32_t auto_log_u(size_t entries, int requested) {
+  if (requested >= 0) return (uint32_t)requested;
+  // aim for ~2^19 work units; units of 8 .. 256 entries
+  int lu = 0;
+  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 8) lu++;
+  return (uint32_t)std::max(lu, 3);
+}
+
+static uint32_t auto_log_g(const MsmLayout &L, int requested) {
+  if (requested >= 0) return (uint32_t)requested;
+  uint32_t nbmax = 0;
+  for (int w = 0; w < L.W; w++) nbmax = std::max(nbmax, L.nb[w]);
+  // ~4096+ groups in total, at most 64 buckets per group, at least 4
+  uint32_t lg = 2;
+  while (lg < 6 && ((uint64_t)L.K >> lg) > 16384) lg++;
+  return lg;
+}
+
+}  // namespace zp
+
+using namespace zp;
+
+struct NonConv {
+  uint64_t idx;
+  F x, y;  // Weierstrass affine, R' Montgomery form
+};
+
+struct zp_msm_context {
+  size_t n = 0;
+  MsmOptions opts;
+  hipStream_t stream = nullptr;
+  PackedPre *d_pts = nullptr;
+  uint8_t *d_conv = nullptr;
+  std::vector<NonConv> nonconv;
+  Workspace ws;
+  std::vector<Xyzt> h_winsum;
+};
+
+// ------------------------------------------------------------ finalize --
+// Window combine (driver.cpp:340-347 as Horner), conversion to affine
+// Weierstrass (bls12_377_g1.h:427-439) and the non-convertible fallback
+// (driver.cpp:349-372), then the wire form (x R, y R, R) or z = 0.
+static void finalize(const MsmLayout &L, const Xyzt *win, const std::vector<NonConv> &nonconv,
+                     const biginteger256_t *scalars_host_batch, g1_projective_t *out) {
+  const FConsts &K = consts();
+  Xyzt acc = win[L.W - 1];
+  for (int w = L.W - 2; w >= 0; w--) {
+    for (int i = 0; i < L.width[w]; i++) ec_dbl(acc, K);
+    ec_add(acc, win[w], K);
+  }
+  Jac res;
+  res.Z = f_zero();
+  res.X = f_zero();
+  res.Y = K.one;
+  const F den = f_mul(f_sub_kq(acc.z, acc.y, K.q2), acc.x);
+  if (!f_is_zero_canon(acc.z) && !f_is_zero_canon(den)) {
+    const F inv = f_inv(den, K.one);
+    const F zpy = f_add(acc.z, acc.y);
+    const F u = f_mul(f_mul(zpy, acc.x), inv);
+    const F v = f_mul(f_mul(f_mul(zpy, acc.z), inv), K.ts);
+    res.X = f_canonical(f_add(f_mul(u, K.inv_cb), K.ca_3cb));
+    res.Y = f_canonical(f_mul(v, K.inv_cb));
+    res.Z = K.one;
+  }
+  for (const NonConv &nc : nonconv) {
+    const biginteger256_t &k = scalars_host_batch[nc.idx];
+    Jac kp;
+    kp.X = f_zero();
+    kp.Y = K.one;
+    kp.Z = f_zero();
+    for (int bit = 252; bit >= 0; bit--) {
+      jac_dbl(kp, K);
+      if ((k.data[bit >> 6] >> (bit & 63)) & 1) jac_add_aff(kp, nc.x, nc.y, K);
+    }
+    jac_add_jac(res, kp, K);
+  }
+  memset(out, 0, sizeof(*out));
+  if (f_is_zero_canon(res.Z)) return;  // infinity: z = 0
+  const F zi = f_inv(res.Z, K.one), zi2 = f_sqr(zi);
+  const F x = f_mul(res.X, zi2), y = f_mul(res.Y, f_mul(zi2, zi));
+  const U384 wx = u384_from_f(f_mul(x, K.to_wire));
+  const U384 wy = u384_from_f(f_mul(y, K.to_wire));
+  const U384 wz = u384_from_f(f_mul(K.one, K.to_wire));
+  memcpy(out->x.data, wx.w, 48);
+  memcpy(out->y.data, wy.w, 48);
+  memcpy(out->z.data, wz.w, 48);
+}
+
+// ------------------------------------------------------------ pipeline --
+static int run_msm(zp_msm_context *ctx, size_t n, const uint64_t *d_scalars, const biginteger256_t *h_scalars,
+                   g1_projective_t *out, hipStream_t stream) {
+  Workspace &ws = ctx->ws;
+  const MsmLayout L = choose_layout(ctx->opts, n);
+  const size_t entries = n * (size_t)L.W;
+  const uint32_t log_u = auto_log_u(entries, ctx->opts.log_u);
+  const uint32_t log_g = auto_log_g(L, ctx->opts.log_g);
+  ZP_TRY(ensure_workspace(ws, L, log_u, log_g, n, 0, stream));
+  const FConsts &K = consts();
+  const uint32_t KB = L.K;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(KB + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)KB + 1, 0u);
+  ZP_HIP(hipGetLastError());
+  if (n) {
+    hipLaunchKernelGGL(k_digits_count, dim3(grid_for(n)), dim3(MSM_THREADS), 0, stream, d_scalars, ctx->d_conv,
+                       (uint64_t)n, L, ws.cnt);
+    ZP_HIP(hipGetLastError());
+  }
+  size_t tb = ws.scan_tmp_bytes;
+  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.cnt, ws.off, 0u, KB + 1, rocprim::plus<uint32_t>(), stream));
+  ZP_HIP(hipMemcpyAsync(ws.cursor, ws.off, (KB + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+  if (n) {
+    hipLaunchKernelGGL(k_digits_scatter, dim3(grid_for(n)), dim3(MSM_THREADS), 0, stream, d_scalars, ctx->d_conv,
+                       (uint64_t)n, L, ws.cursor, ws.vals);
+    ZP_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(KB + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, KB, log_u,
+                     ws.nunits);
+  ZP_HIP(hipGetLastError());
+  tb = ws.scan_tmp_bytes;
+  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, KB + 1, rocprim::plus<uint32_t>(),
+                                 stream));
+  hipLaunchKernelGGL(k_fill_units, dim3(grid_for(KB)), dim3(MSM_THREADS), 0, stream, ws.nunits, ws.ustart, KB,
+                     ws.ubucket);
+  ZP_HIP(hipGetLastError());
+  const size_t max_units = (entries >> log_u) + KB + 1;
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_units, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream,
+                     ctx->d_pts, ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, ws.ustart + KB, log_u, ws.partial, K);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(KB)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart,
+                     ws.nunits, KB, ws.buckets, K);
+  ZP_HIP(hipGetLastError());
+  GroupLayout G{};
+  G.log_g = log_g;
+  uint32_t gs = 0;
+  for (int w = 0; w < L.W; w++) {
+    G.gstart[w] = gs;
+    gs += (L.nb[w] + (1u << log_g) - 1) >> log_g;
+  }
+  G.gstart[L.W] = gs;
+  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, L, G, ws.groups,
+                     K);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_window_sum, dim3(L.W), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups, G,
+                     ws.winsum, K);
+  ZP_HIP(hipGetLastError());
+  ctx->h_winsum.resize(L.W);
+  ZP_HIP(hipMemcpyAsync(ctx->h_winsum.data(), ws.winsum, L.W * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
+  ZP_HIP(hipStreamSynchronize(stream));
+  // Scalars of the non-convertible points (host copy when only device scalars exist).
+  std::vector<biginteger256_t> nc_scalars;
+  const biginteger256_t *hs = h_scalars;
+  std::vector<NonConv> nc_in_prefix;
+  for (const NonConv &nc : ctx->nonconv)
+    if (nc.idx < n) nc_in_prefix.push_back(nc);
+  if (!nc_in_prefix.empty() && !hs) {
+    nc_scalars.assign(n, biginteger256_t{{0, 0, 0, 0}});
+    for (const NonConv &nc : nc_in_prefix)
+      ZP_HIP(hipMemcpy(&nc_scalars[nc.idx], d_scalars + 4 * nc.idx, 32, hipMemcpyDeviceToHost));
+    hs = nc_scalars.data();
+  }
+  finalize(L, ctx->h_winsum.data(), nc_in_prefix, hs, out);
+  return ZP_OK;
+}
+
+// ---------------------------------------------------------------- init --
+static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_device) {
+  const FConsts &K = consts();
+  const size_t n = ctx->n;
+  ZP_HIP(hipMalloc(&ctx->d_pts, std::max<size_t>(n, 1) * sizeof(PackedPre)));
+  ZP_HIP(hipMalloc(&ctx->d_conv, std::max<size_t>(n, 1)));
+  const size_t SLICE = std::min<size_t>(n ? n : 1, (size_t)1 << 21);
+  g1_affine_t *d_stage = nullptr;
+  F *d_u = nullptr, *d_v = nullptr, *d_den = nullptr, *d_tmp = nullptr;
+  uint8_t *d_flag = nullptr;
+  int st = ZP_OK;
+  auto fail = [&](const char *what) {
+    set_last_error("msm_init: %s", what);
+    st = ZP_ERR_OUT_OF_MEMORY;
+  };
+  if (!on_device && hipMalloc(&d_stage, SLICE * sizeof(g1_affine_t)) != hipSuccess) fail("staging allocation");
+  if (st == ZP_OK && (hipMalloc(&d_u, SLICE * sizeof(F)) != hipSuccess || hipMalloc(&d_v, SLICE * sizeof(F)) != hipSuccess ||
+                      hipMalloc(&d_den, SLICE * sizeof(F)) != hipSuccess ||
+                      hipMalloc(&d_tmp, SLICE * sizeof(F)) != hipSuccess || hipMalloc(&d_flag, SLICE) != hipSuccess))
+    fail("temporary allocation");
+  hipStream_t s = ctx->stream;
+  for (size_t b = 0; st == ZP_OK && b < n; b += SLICE) {
+    const size_t m = std::min(SLICE, n - b);
+    const g1_affine_t *src = points + b;
+    if (!on_device) {
+      if (hipMemcpyAsync(d_stage, points + b, m * sizeof(g1_affine_t), hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_last_error("msm_init: point upload failed");
+        st = ZP_ERR_HIP;
+        break;
+      }
+      src = d_stage;
+    }
+    hipLaunchKernelGGL(k_pre_a, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, src, (uint64_t)m, K, d_u, d_v, d_den,
+                       d_flag);
+    const uint32_t chunk = 32;
+    hipLaunchKernelGGL(k_batch_inv, dim3(grid_for((m + chunk - 1) / chunk, 64)), dim3(64), 0, s, d_den, (uint64_t)m,
+                       chunk, d_tmp, K);
+    hipLaunchKernelGGL(k_pre_c, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, d_u, d_v, d_den, d_flag, (uint64_t)m, K,
+                       ctx->d_pts + b, ctx->d_conv + b);
+    if (hipGetLastError() != hipSuccess) {
+      set_last_error("msm_init: precompute launch failed");
+      st = ZP_ERR_HIP;
+    }
+  }
+  if (st == ZP_OK && hipStreamSynchronize(s) != hipSuccess) {
+    set_last_error("msm_init: precompute failed");
+    st = ZP_ERR_HIP;
+  }
+  if (d_stage) (void)hipFree(d_stage);
+  if (d_u) (void)hipFree(d_u);
+  if (d_v) (void)hipFree(d_v);
+  if (d_den) (void)hipFree(d_den);
+  if (d_tmp) (void)hipFree(d_tmp);
+  if (d_flag) (void)hipFree(d_flag);
+  if (st != ZP_OK) return st;
+  // Non-convertible points (driver.cpp:160-172): kept on the host.
+  std::vector<uint8_t> conv(n);
+  if (n) ZP_HIP(hipMemcpy(conv.data(), ctx->d_conv, n, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; i++) {
+    if (conv[i]) continue;
+    g1_affine_t p;
+    if (on_device)
+      ZP_HIP(hipMemcpy(&p, points + i, sizeof(p), hipMemcpyDeviceToHost));
+    else
+      p = points[i];
+    NonConv nc;
+    nc.idx = i;
+    U384 x, y;
+    memcpy(x.w, p.x.data, 48);
+    memcpy(y.w, p.y.data, 48);
+    nc.x = f_canonical(f_mul(f_from_u384(x), K.to_int));
+    nc.y = f_canonical(f_mul(f_from_u384(y), K.to_int));
+    ctx->nonconv.push_back(nc);
+  }
+  return ZP_OK;
+}
+
+static int msm_init_common(const char *options, size_t options_len, const g1_affine_t *points, size_t npoints,
+                           bool on_device, zp_msm_context **out) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "msm_init: out is NULL");
+  *out = nullptr;
+  ZP_CHECK(points != nullptr || npoints == 0, ZP_ERR_INVALID_ARG, "msm_init: points is NULL");
+  ZP_CHECK(npoints < (1ull << 31), ZP_ERR_TOO_LARGE, "msm_init: %zu points > 2^31 - 1", npoints);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_last_error("msm_init: no HIP device");
+    return ZP_ERR_NO_DEVICE;
+  }
+  const MsmOptions opts = parse_options(options, options_len);
+  ZP_CHECK(opts.window == 0 || (opts.window >= 8 && opts.window <= 24), ZP_ERR_INVALID_ARG,
+           "msm_init: window=%d not in [8, 24]", opts.window);
+  ZP_CHECK(opts.log_u <= 12 && opts.log_g <= 10, ZP_ERR_INVALID_ARG, "msm_init: log_u/log_g out of range");
+  zp_msm_context *ctx = new zp_msm_context();
+  ctx->n = npoints;
+  ctx->opts = opts;
+  int st = ZP_OK;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_last_error("msm_init: stream creation failed");
+    st = ZP_ERR_HIP;
+  }
+  if (st == ZP_OK) st = init_points(ctx, points, on_device);
+  if (st != ZP_OK) {
+    msm_free(ctx);
+    return st;
+  }
+  *out = ctx;
+  return ZP_OK;
+}
+
+extern "C" int zp_msm_init(const char *options, size_t options_len, const g1_affine_t *points, size_t npoints,
+                           zp_msm_context **out) {
+  return msm_init_common(options, options_len, points, npoints, false, out);
+}
+
+extern "C" int zp_msm_init_device(const char *options, size_t options_len, const g1_affine_t *d_points,
+                                  size_t npoints, zp_msm_context **out) {
+  return msm_init_common(options, options_len, d_points, npoints, true, out);
+}
+
+extern "C" zp_msm_context *msm_init(const char *xclbin, ssize_t xclbin_len, g1_affine_t *rust_points,
+                                    ssize_t npoints) {
+  zp_msm_context *ctx = nullptr;
+  if (npoints < 0 || xclbin_len < 0) {
+    set_last_error("msm_init: negative length");
+    return nullptr;
+  }
+  if (zp_msm_init(xclbin, (size_t)xclbin_len, rust_points, (size_t)npoints, &ctx) != ZP_OK) {
+    fprintf(stderr, "msm_init failed: %s\n", zp_last_error());
+    return nullptr;
+  }
+  return ctx;
+}
+
+extern "C" void msm_free(zp_msm_context *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->ws.release();
+  if (ctx->d_pts) (void)hipFree(ctx->d_pts);
+  if (ctx->d_conv) (void)hipFree(ctx->d_conv);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches,
+                           const biginteger256_t *scalars) {
+  ZP_CHECK(ctx != nullptr && out != nullptr && (scalars != nullptr || num_batches == 0 || ctx->n == 0),
+           ZP_ERR_INVALID_ARG, "zp_msm_mult: NULL argument");
+  const size_t n = ctx->n;
+  size_t cap = ctx->ws.cap_scalars;
+  ZP_TRY(grow(ctx->ws.scalars, cap, std::max<size_t>(4 * n, 4)));
+  ctx->ws.cap_scalars = cap;
+  for (uint64_t b = 0; b < num_batches; b++) {
+    const biginteger256_t *hs = scalars + b * n;
+    if (n) ZP_HIP(hipMemcpyAsync(ctx->ws.scalars, hs, n * 32, hipMemcpyHostToDevice, ctx->stream));
+    ZP_TRY(run_msm(ctx, n, ctx->ws.scalars, hs, out + b, ctx->stream));
+  }
+  return ZP_OK;
+}
+
+extern "C" void msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches,
+                         biginteger256_t *ptr_scalars) {
+  if (zp_msm_mult(ctx, out, num_batches, ptr_scalars) != ZP_OK)
+    fprintf(stderr, "msm_mult failed: %s\n", zp_last_error());
+}
+
+extern "C" int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches,
+                                  const biginteger256_t *d_scalars, void *stream) {
+  ZP_CHECK(ctx != nullptr && out != nullptr && (d_scalars != nullptr || num_batches == 0 || ctx->n == 0),
+           ZP_ERR_INVALID_ARG, "zp_msm_mult_device: NULL argument");
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  for (uint64_t b = 0; b < num_batches; b++)
+    ZP_TRY(run_msm(ctx, ctx->n, reinterpret_cast<const uint64_t *>(d_scalars + b * ctx->n), nullptr, out + b, s));
+  return ZP_OK;
+}
+
+extern "C" int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_projective_t *out,
+                                         const biginteger256_t *d_scalars, void *stream) {
+  ZP_CHECK(ctx != nullptr && out != nullptr, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_device: NULL argument");
+  ZP_CHECK(n <= ctx->n, ZP_ERR_TOO_LARGE, "zp_msm_mult_prefix_device: n %zu > %zu points", n, ctx->n);
+  ZP_CHECK(d_scalars != nullptr || n == 0, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_device: NULL scalars");
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  return run_msm(ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), nullptr, out, s);
+}
+
+extern "C" size_t zp_msm_num_points(const zp_msm_context *ctx) { return ctx ? ctx->n : 0; }
+extern "C" size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx) { return ctx ? ctx->nonconv.size() : 0; }
+
+// ---------------------------------------------------------- generator --
+static std::mutex g_gen_mu;
+static std::map<int, PackedPre *> g_gen_table;  // device -> 32 x 256 comb table
+
+static int gen_table(PackedPre **out) {
+  int dev = 0;
+  ZP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_gen_mu);
+  auto it = g_gen_table.find(dev);
+  if (it != g_gen_table.end()) {
+    *out = it->second;
+    return ZP_OK;
+  }
+  const FConsts &K = consts();
+  // Generator in twisted-Edwards affine form via the import path.
+  const F r2 = f_from_u384(pow2_mod_q(784));
+  const F gx = f_mul(f_from_u384(u384_from_hex(HEX_GX)), r2);
+  const F gy = f_mul(f_from_u384(u384_from_hex(HEX_GY)), r2);
+  const F u = f_mul(f_sub_kq(gx, K.alpha, K.q2), K.mont_s);
+  const F v = f_mul(K.mont_s, gy);
+  const F tx = f_mul(f_mul(K.ts, u), f_inv(v, K.one));
+  const F ty = f_mul(f_sub_kq(u, K.one, K.q2), f_inv(f_add(u, K.one), K.one));
+  Xyzt base;
+  base.x = tx;
+  base.y = ty;
+  base.z = K.one;
+  base.t = f_mul(tx, ty);
+  std::vector<Xyzt> pts(32 * 256);
+  for (int j = 0; j < 32; j++) {
+    pts[j * 256] = ec_identity(K);
+    pts[j * 256 + 1] = base;
+    for (int m = 2; m < 256; m++) {
+      Xyzt t = pts[j * 256 + m - 1];
+      ec_add(t, base, K);
+      pts[j * 256 + m] = t;
+    }
+    for (int d = 0; d < 8; d++) ec_dbl(base, K);
+  }
+  // Batch-normalise to affine and precompute.
+  std::vector<F> pref(pts.size());
+  F acc = K.one;
+  for (size_t i = 0; i < pts.size(); i++) {
+    acc = f_mul(acc, pts[i].z);
+    pref[i] = acc;
+  }
+  F inv = f_inv(acc, K.one);
+  std::vector<PackedPre> table(pts.size());
+  for (size_t i = pts.size(); i-- > 0;) {
+    const F zi = f_mul(inv, i ? pref[i - 1] : K.one);
+    inv = f_mul(inv, pts[i].z);
+    const F x = f_mul(pts[i].x, zi), y = f_mul(pts[i].y, zi);
+    table[i] = pre_pack(pre_from_affine(x, y, K));
+  }
+  PackedPre *d = nullptr;
+  ZP_HIP(hipMalloc(&d, table.size() * sizeof(PackedPre)));
+  ZP_HIP(hipMemcpy(d, table.data(), table.size() * sizeof(PackedPre), hipMemcpyHostToDevice));
+  g_gen_table[dev] = d;
+  *out = d;
+  return ZP_OK;
+}
+
+extern "C" int zp_g1_mul_generator_device(const biginteger256_t *d_k, g1_affine_t *d_out, size_t n, void *stream) {
+  ZP_CHECK((d_k && d_out) || n == 0, ZP_ERR_INVALID_ARG, "zp_g1_mul_generator_device: NULL argument");
+  if (n == 0) return ZP_OK;
+  PackedPre *table = nullptr;
+  ZP_TRY(gen_table(&table));
+  const FConsts &K = consts();
+  hipStream_t s = (hipStream_t)stream;
+  const size_t SLICE = std::min<size_t>(n, (size_t)1 << 21);
+  Xyzt *d_acc = nullptr;
+  F *d_den = nullptr, *d_tmp = nullptr;
+  uint8_t *d_flag = nullptr;
+  int st = ZP_OK;
+  if (hipMalloc(&d_acc, SLICE * sizeof(Xyzt)) != hipSuccess || hipMalloc(&d_den, SLICE * sizeof(F)) != hipSuccess ||
+      hipMalloc(&d_tmp, SLICE * sizeof(F)) != hipSuccess || hipMalloc(&d_flag, SLICE) != hipSuccess) {
+    set_last_error("zp_g1_mul_generator_device: allocation failed");
+    st = ZP_ERR_OUT_OF_MEMORY;
+  }
+  for (size_t b = 0; st == ZP_OK && b < n; b += SLICE) {
+    const size_t m = std::min(SLICE, n - b);
+    hipLaunchKernelGGL(k_gen_a, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, d_k + b, (uint64_t)m, table, K, d_acc,
+                       d_den, d_flag);
+    const uint32_t chunk = 32;
+    hipLaunchKernelGGL(k_batch_inv, dim3(grid_for((m + chunk - 1) / chunk, 64)), dim3(64), 0, s, d_den, (uint64_t)m,
+                       chunk, d_tmp, K);
+    hipLaunchKernelGGL(k_gen_c, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, d_acc, d_den, d_flag, (uint64_t)m, K,
+                       d_out + b);
+    if (hipGetLastError() != hipSuccess) {
+      set_last_error("zp_g1_mul_generator_device: launch failed");
+      st = ZP_ERR_HIP;
+    }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess && st == ZP_OK) {
+    set_last_error("zp_g1_mul_generator_device: execution failed");
+    st = ZP_ERR_HIP;
+  }
+  if (d_acc) (void)hipFree(d_acc);
+  if (d_den) (void)hipFree(d_den);
+  if (d_tmp) (void)hipFree(d_tmp);
+  if (d_flag) (void)hipFree(d_flag);
+  return st;
+}
+
+// -------------------------------------------------------- diagnostics --
+extern "C" int zp_debug_fq_mul(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n) {
+  ZP_CHECK((a && b && out) || n == 0, ZP_ERR_INVALID_ARG, "zp_debug_fq_mul: NULL argument");
+  if (!n) return ZP_OK;
+  uint64_t *d = nullptr;
+  ZP_HIP(hipMalloc(&d, 3 * n * 48));
+  ZP_HIP(hipMemcpy(d, a, n * 48, hipMemcpyHostToDevice));
+  ZP_HIP(hipMemcpy(d + 6 * n, b, n * 48, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_debug_mul, dim3(grid_for(n)), dim3(MSM_THREADS), 0, 0, d, d + 6 * n, d + 12 * n, (uint64_t)n,
+                     consts());
+  ZP_HIP(hipGetLastError());
+  ZP_HIP(hipMemcpy(out, d + 12 * n, n * 48, hipMemcpyDeviceToHost));
+  ZP_HIP(hipFree(d));
+  return ZP_OK;
+}
+
+extern "C" int zp_debug_msm_digits(const char *options, size_t options_len, const biginteger256_t *scalars, size_t n,
+                                   int32_t *digits, int *num_windows) {
+  ZP_CHECK(num_windows != nullptr, ZP_ERR_INVALID_ARG, "zp_debug_msm_digits: NULL num_windows");
+  const MsmOptions opts = parse_options(options, options_len);
+  ZP_CHECK(opts.window == 0 || (opts.window >= 8 && opts.window <= 24), ZP_ERR_INVALID_ARG,
+           "zp_debug_msm_digits: window=%d not in [8, 24]", opts.window);
+  const MsmLayout L = choose_layout(opts, n);
+  *num_windows = L.W;
+  if (!digits || !n) return ZP_OK;
+  ZP_CHECK(scalars != nullptr, ZP_ERR_INVALID_ARG, "zp_debug_msm_digits: NULL scalars");
+  uint64_t *ds = nullptr;
+  int32_t *dd = nullptr;
+  ZP_HIP(hipMalloc(&ds, n * 32));
+  ZP_HIP(hipMalloc(&dd, n * L.W * sizeof(int32_t)));
+  ZP_HIP(hipMemcpy(ds, scalars, n * 32, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_digits_dump, dim3(grid_for(n)), dim3(MSM_THREADS), 0, 0, ds, (uint64_t)n, L, dd);
+  ZP_HIP(hipGetLastError());
+  ZP_HIP(hipMemcpy(digits, dd, n * L.W * sizeof(int32_t), hipMemcpyDeviceToHost));
+  ZP_HIP(hipFree(ds));
+  ZP_HIP(hipFree(dd));
+  return ZP_OK;
+}
+
+extern "C" int zp_debug_msm_precomputed(zp_msm_context *ctx, size_t i0, size_t count, uint64_t *out) {
+  ZP_CHECK(ctx != nullptr && (out != nullptr || count == 0), ZP_ERR_INVALID_ARG, "zp_debug_msm_precomputed: NULL");
+  ZP_CHECK(i0 + count <= ctx->n, ZP_ERR_INVALID_ARG, "zp_debug_msm_precomputed: range beyond %zu points", ctx->n);
+  if (!count) return ZP_OK;
+  uint64_t *d = nullptr;
+  ZP_HIP(hipMalloc(&d, count * 18 * 8));
+  hipLaunchKernelGGL(k_debug_pre_wire, dim3(grid_for(count)), dim3(MSM_THREADS), 0, 0, ctx->d_pts, (uint64_t)i0,
+                     (uint64_t)count, consts(), d);
+  ZP_HIP(hipGetLastError());
+  ZP_HIP(hipMemcpy(out, d, count * 18 * 8, hipMemcpyDeviceToHost));
+  ZP_HIP(hipFree(d));
+  return ZP_OK;
+}

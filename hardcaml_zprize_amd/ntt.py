@@ -1,0 +1,122 @@
+"""Goldilocks NTT host driver over the C ABI.
+
+`NttGpuDriver` keeps the method names, argument meaning and error behaviour of
+the reference's `NttFpgaDriver` (zprize/ntt/host/driver.h:58-225):
+request_buffer returns None when every buffer is in flight, free_buffer of a
+free buffer raises, simple_evaluate zero-pads short input and raises when the
+input is longer than the transform.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import ZpError, call, lib
+
+FORWARD, INVERSE = 0, 1
+
+
+class UserBuffer:
+    """One in-flight buffer (driver.h:62-93): pinned host input/output vectors."""
+
+    def __init__(self, driver, buffer_id):
+        self.driver = driver
+        self.id = buffer_id
+        n = driver.input_vector_size()
+        self._in = _host_view(lib().zp_ntt_buffer_input(driver.plan, buffer_id), n)
+        self._out = _host_view(lib().zp_ntt_buffer_output(driver.plan, buffer_id), n)
+
+    def input_data(self):
+        return self._in
+
+    def output_data(self):
+        return self._out
+
+
+def _host_view(addr, n):
+    if not addr:
+        raise RuntimeError("buffer pointer is NULL")
+    buf = (ctypes.c_uint64 * n).from_address(addr)
+    return np.frombuffer(buf, dtype=np.uint64, count=n)
+
+
+class NttGpuDriver:
+    """NttFpgaDriver(create_ntt_2_<log_n>) + load_xclbin in one constructor."""
+
+    def __init__(self, log_n, num_buffers=8):
+        h = ctypes.c_void_p()
+        call("zp_ntt_create", log_n, num_buffers, ctypes.byref(h))
+        self.plan = h
+        self.log_n = log_n
+
+    def close(self):
+        if self.plan:
+            lib().zp_ntt_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def input_vector_size(self):
+        return int(lib().zp_ntt_size(self.plan))
+
+    def max_num_buffers_in_flight(self):
+        return int(lib().zp_ntt_max_buffers_in_flight(self.plan))
+
+    def request_buffer(self):
+        bid = ctypes.c_int(-1)
+        st = lib().zp_ntt_request_buffer(self.plan, ctypes.byref(bid))
+        if st == 4:  # ZP_ERR_NO_BUFFER: the reference returns NULL
+            return None
+        if st != 0:
+            call("zp_ntt_request_buffer", st)
+        return UserBuffer(self, bid.value)
+
+    def transfer_input_vector_to_user_buffer(self, buf, data):
+        data = np.asarray(data, dtype=np.uint64)
+        buf.input_data()[:] = data
+
+    def enqueue_evaluation_async(self, buf, direction=FORWARD):
+        call("zp_ntt_enqueue", self.plan, buf.id, direction)
+
+    def wait_for_result(self, buf):
+        call("zp_ntt_wait", self.plan, buf.id)
+
+    def transfer_user_buffer_to_output_vector(self, buf, out=None):
+        if out is None:
+            return buf.output_data().copy()
+        out[:] = buf.output_data()
+        return out
+
+    def free_buffer(self, buf):
+        call("zp_ntt_free_buffer", self.plan, buf.id)
+
+    def simple_evaluate(self, data, direction=FORWARD):
+        data = np.ascontiguousarray(data, dtype=np.uint64)
+        out = np.empty(self.input_vector_size(), dtype=np.uint64)
+        call("zp_ntt_simple_evaluate", self.plan, out.ctypes.data, data.ctypes.data, len(data), direction)
+        return out
+
+    def inplace_device(self, d_ptr, direction=FORWARD, stream=None):
+        """The hot path: in-place transform of n u64 already in HBM."""
+        call("zp_ntt_inplace_device", self.plan, ctypes.c_void_p(d_ptr), direction, ctypes.c_void_p(stream or 0))
+
+
+def ntt_batch_device(ptrs, log_ns, dirs, stream=None):
+    """Many in-place transforms of mixed sizes/directions, one launch sequence."""
+    n = len(ptrs)
+    P = (ctypes.c_void_p * n)(*ptrs)
+    L = (ctypes.c_uint * n)(*log_ns)
+    D = (ctypes.c_int * n)(*dirs)
+    call("zp_ntt_batch_device", P, L, D, n, ctypes.c_void_p(stream or 0))
+
+
+__all__ = ["NttGpuDriver", "UserBuffer", "ntt_batch_device", "FORWARD", "INVERSE", "ZpError"]

@@ -1,0 +1,181 @@
+/*
+ * zpgpu: MI355X (gfx950) Goldilocks NTT and BLS12-377 G1 MSM behind a C ABI.
+ *
+ * Drop-in boundary for the two hot paths of fyquah/hardcaml_zprize.  Every
+ * entry point names the reference interface it replaces (paths relative to
+ * the reference checkout).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Functions returning int return a zp_status; zp_last_error() gives the
+ *     text of the calling thread's last failure.
+ *   - "d_" pointers are device (HBM) pointers; "stream" is a hipStream_t
+ *     passed as void* (NULL = the null stream).
+ *   - One handle is not reentrant; distinct handles may be used from
+ *     distinct threads.
+ */
+#ifndef ZPGPU_H
+#define ZPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <sys/types.h>
+
+#ifdef __cplusplus
+#define ZP_BOOL bool
+extern "C" {
+#else
+#include <stdbool.h>
+#define ZP_BOOL bool
+#endif
+
+typedef enum {
+  ZP_OK = 0,
+  ZP_ERR_INVALID_ARG = 1, /* bad size / pointer / direction */
+  ZP_ERR_HIP = 2,         /* a HIP runtime call failed */
+  ZP_ERR_OUT_OF_MEMORY = 3,
+  ZP_ERR_NO_BUFFER = 4,   /* all in-flight buffers are in use */
+  ZP_ERR_STATE = 5,       /* call out of order (e.g. free of a free buffer) */
+  ZP_ERR_TOO_LARGE = 6,   /* data_length > transform size, budget exceeded */
+  ZP_ERR_NO_DEVICE = 7
+} zp_status;
+
+typedef enum { ZP_NTT_FORWARD = 0, ZP_NTT_INVERSE = 1 } zp_ntt_dir;
+
+const char *zp_last_error(void);
+/* Library / kernel build identification, e.g. "zpgpu gfx950". */
+const char *zp_version(void);
+
+/* ------------------------------------------------------------------ NTT --
+ * Goldilocks p = 2^64 - 2^32 + 1, natural-order input and output, canonical
+ * values in [0, p).  FORWARD: X_k = sum_j a_j w^(jk), w = 7^((p-1)/n), the
+ * transform of zprize/ntt/host/reference.cpp:87-152 (the hardware core's).
+ * INVERSE: n^-1 * sum_j X_j w^(-jk), so INVERSE(FORWARD(a)) == a.
+ */
+typedef struct zp_ntt_plan zp_ntt_plan;
+
+/* NttFpgaDriver(NttFpgaDriverArg::create_ntt_2_24 / _2_18 / _2_12) +
+ * load_xclbin (zprize/ntt/host/driver.h:110-134, driver.cpp:107-138,364-454).
+ * log_n in [1, 26].  Allocates twiddle tables, one device scratch vector and
+ * `num_buffers` (<= 64; 0 means 8, driver.cpp:133) in-flight user buffers. */
+int zp_ntt_create(unsigned log_n, unsigned num_buffers, zp_ntt_plan **out);
+int zp_ntt_destroy(zp_ntt_plan *plan);
+/* NttFpgaDriver::input_vector_size (driver.h:127). */
+uint64_t zp_ntt_size(const zp_ntt_plan *plan);
+/* NttFpgaDriver::max_num_buffers_in_flight (driver.h:142-144). */
+unsigned zp_ntt_max_buffers_in_flight(const zp_ntt_plan *plan);
+
+/* The hot path: transform n u64 resident in HBM, in place, on `stream`
+ * (the device work of enqueue_phase1_work + enqueue_phase2_work,
+ * zprize/ntt/host/driver.cpp:201-231).  Asynchronous. */
+int zp_ntt_inplace_device(zp_ntt_plan *plan, uint64_t *d_data, int dir, void *stream);
+
+/* NttFpgaDriver::simple_evaluate (driver.h:188-197, driver.cpp:332-344):
+ * host in, host out, blocking; input shorter than n is zero padded;
+ * data_length > n is ZP_ERR_TOO_LARGE (driver.cpp:289-297). */
+int zp_ntt_simple_evaluate(zp_ntt_plan *plan, uint64_t *out, const uint64_t *in, uint64_t data_length,
+                           int dir);
+
+/* Asynchronous buffer API, NttFpgaDriver::request_buffer / transfer_* /
+ * enqueue_evaluation_async / wait_for_result / free_buffer
+ * (driver.h:146-186, driver.cpp:148-286).  Buffers are ids in
+ * [0, max_buffers_in_flight).  Input and output are pinned host vectors. */
+int zp_ntt_request_buffer(zp_ntt_plan *plan, int *buffer_id); /* ZP_ERR_NO_BUFFER when all busy */
+uint64_t *zp_ntt_buffer_input(zp_ntt_plan *plan, int buffer_id);
+uint64_t *zp_ntt_buffer_output(zp_ntt_plan *plan, int buffer_id);
+int zp_ntt_enqueue(zp_ntt_plan *plan, int buffer_id, int dir);
+int zp_ntt_wait(zp_ntt_plan *plan, int buffer_id);
+int zp_ntt_free_buffer(zp_ntt_plan *plan, int buffer_id); /* ZP_ERR_STATE if not in use */
+
+/* Batched form: `count` transforms of mixed sizes and directions, all
+ * resident in HBM, in place, issued as one launch sequence on `stream`
+ * (captured once into a HIP graph per distinct job list when use_graph != 0).
+ * log_ns[i] in [1, 24]; d_ptrs[i] device pointers; dirs[i] in {0,1}.
+ * No reference counterpart: the reference runs one transform per enqueue
+ * (bench_throughput.cpp:61-107). */
+int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
+                        void *stream);
+
+/* ------------------------------------------------------------------ MSM --
+ * Layouts of zprize/msm_pippenger/host/msm_compute/include/bls12_377_g1/rust_types.h:8-83.
+ * Base-field words are little-endian 64-bit limbs in Montgomery form with
+ * R = 2^384 (the arkworks 0.3 representation); scalars are plain 256-bit
+ * little-endian integers < r.
+ */
+typedef struct { uint64_t data[4]; } biginteger256_t;
+typedef struct { uint64_t data[6]; } biginteger384_t;
+typedef struct { biginteger384_t x, y; ZP_BOOL infinity; } g1_affine_t;   /* 104 bytes */
+typedef struct { biginteger384_t x, y, z; } g1_projective_t;             /* 144 bytes */
+
+typedef struct zp_msm_context zp_msm_context;
+
+/* extern "C" Driver *msm_init(const char *xclbin, ssize_t xclbin_len,
+ *                             g1_affine_t *rust_points, ssize_t npoints)
+ * (zprize/msm_pippenger/host/driver/driver.cpp:603-617).  The xclbin string
+ * (not NUL terminated) is an optional option string here, e.g. "window=16";
+ * it may be NULL/empty.  Points are converted on the GPU to the
+ * twisted-Edwards precomputed form and stay resident in HBM.  Returns NULL
+ * on failure (zp_last_error() says why). */
+zp_msm_context *msm_init(const char *xclbin, ssize_t xclbin_len, g1_affine_t *rust_points, ssize_t npoints);
+
+/* extern "C" void msm_mult(Driver *driver, g1_projective_t *out,
+ *                          uint64_t num_batches, biginteger256_t *ptr_scalars)
+ * (driver.cpp:619-636).  scalars[b * npoints + i]; out[b] = (x*R, y*R, R)
+ * for an affine result (x, y), or z = 0 for the point at infinity. */
+void msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches, biginteger256_t *ptr_scalars);
+
+/* Not in the reference (its Driver is never freed): releases the context. */
+void msm_free(zp_msm_context *ctx);
+
+/* Status-returning forms of the two entry points above. */
+int zp_msm_init(const char *options, size_t options_len, const g1_affine_t *points, size_t npoints,
+                zp_msm_context **out);
+/* msm_init with the points already resident in HBM (d_points: npoints g1_affine_t). */
+int zp_msm_init_device(const char *options, size_t options_len, const g1_affine_t *d_points, size_t npoints,
+                       zp_msm_context **out);
+int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches, const biginteger256_t *scalars);
+/* Scalars already resident in HBM (num_batches * npoints * 32 bytes). */
+int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches,
+                       const biginteger256_t *d_scalars, void *stream);
+/* MSM over the first n points of the context's table (n <= npoints). */
+int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_projective_t *out, const biginteger256_t *d_scalars,
+                              void *stream);
+size_t zp_msm_num_points(const zp_msm_context *ctx);
+/* Number of loaded points without a twisted-Edwards image (handled by the
+ * Weierstrass fallback, driver.cpp:349-372). */
+size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx);
+
+/* Seeded base-point generation on the GPU: out[i] = k_i * G (G the G1
+ * generator), affine Montgomery form; d_k: n scalars (device), d_out: n
+ * g1_affine_t (device).  Test/benchmark input generation for tables too large
+ * to build on the host. */
+int zp_g1_mul_generator_device(const biginteger256_t *d_k, g1_affine_t *d_out, size_t n, void *stream);
+
+/* -------------------------------------------------------- diagnostics --
+ * Component checks through the device code paths (used by the parity tests). */
+/* out[i] = a[i] * b[i] for wire-form (Montgomery R = 2^384) Fq values, 6 words each. */
+int zp_debug_fq_mul(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t n);
+/* Signed window digits of n host scalars under the layout an options string
+ * selects ("window=reference" = the reference's 20 windows,
+ * config_utils.ml:7-41); digits[i * W + w], W written to *num_windows
+ * (digits may be NULL to query W). */
+int zp_debug_msm_digits(const char *options, size_t options_len, const biginteger256_t *scalars, size_t n,
+                        int32_t *digits, int *num_windows);
+/* The resident precomputed points [i0, i0 + count) as wire-form triples
+ * (y - x, y + x, 2d x y) of the twisted-Edwards image, 18 words each. */
+int zp_debug_msm_precomputed(zp_msm_context *ctx, size_t i0, size_t count, uint64_t *out);
+
+/* ------------------------------------------------------------ utilities -- */
+/* Measured issue rate, lane-operations per second over the whole device:
+ * kind 0 = v_mad_u64_u32 (the Fq multiply's multiply-add), kind 1 = a
+ * full-rate 32-bit VALU op.  The MSM roofline peak is kind 0. */
+int zp_probe_valu_rate(int kind, double *lane_ops_per_s);
+int zp_device_malloc(void **ptr, size_t bytes);
+int zp_device_free(void *ptr);
+int zp_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int zp_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int zp_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,226 @@
+"""BLS12-377 G1 MSM on the GPU through the C ABI, bit-exact against the oracle.
+
+Bit-exactness is on the output bytes (x R, y R, R) / z = 0, which are unique
+for a given group element, so every window layout must agree.
+"""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from hardcaml_zprize_amd import msm as zmsm
+
+pytestmark = pytest.mark.gpu
+Q, R = oracle.Q, oracle.R_ORDER
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _words(ints, n):
+    return np.array([oracle.int_to_words(v, n) for v in ints], dtype=np.uint64)
+
+
+def _scalars(rng, count):
+    return _words([rng.randrange(R) for _ in range(count)], 4)
+
+
+def test_fq_mul_vs_oracle():
+    _say("msm fq mul")
+    rng = random.Random(3)
+    vals = [0, 1, 2, Q - 1, Q - 2, 2**376, (1 << 377) - 1 - ((1 << 377) - Q)] + [rng.randrange(Q) for _ in range(2000)]
+    a = [rng.choice(vals) for _ in range(4000)]
+    b = [rng.choice(vals) for _ in range(4000)]
+    wa = np.array([oracle.to_mont(x) for x in a])
+    wb = np.array([oracle.to_mont(x) for x in b])
+    got = zmsm.debug_fq_mul(wa, wb)
+    for i in range(len(a)):
+        assert oracle.from_mont(got[i]) == a[i] * b[i] % Q
+    # and bytewise against the oracle's own Montgomery multiply
+    for i in range(0, len(a), 97):
+        assert np.array_equal(got[i], oracle.fq_binop("orc_fq_mul", wa[i], wb[i]))
+
+
+def test_digits_reference_layout_vs_oracle():
+    """M3: the reference's 20-window signed digits (utils.ml:243-288), bit-exact."""
+    _say("msm digits")
+    rng = random.Random(8)
+    ints = [0, 1, R - 1, (1 << 252), (1 << 13) - 1, 1 << 12, sum(1 << (13 * i + 12) for i in range(13)) % R]
+    ints += [rng.randrange(R) for _ in range(5000)]
+    got = zmsm.debug_digits(_words(ints, 4), "window=reference")
+    assert got.shape == (len(ints), 20)
+    for i, s in enumerate(ints):
+        mag, neg = oracle.msm_digits(s)
+        exp = [-int(m) if n else int(m) for m, n in zip(mag, neg)]
+        assert list(got[i]) == exp, hex(s)
+
+
+def test_digits_uniform_layouts_reconstruct():
+    _say("msm digits uniform")
+    rng = random.Random(9)
+    ints = [rng.randrange(R) for _ in range(500)] + [R - 1, 0, 1]
+    for c in (8, 11, 13, 16, 19):
+        got = zmsm.debug_digits(_words(ints, 4), f"window={c}")
+        for i, s in enumerate(ints):
+            v = sum(int(d) << (c * w) for w, d in enumerate(got[i]))
+            assert v == s
+            assert all(abs(int(d)) <= (1 << (c - 1)) for d in got[i][:-1])
+
+
+def test_generator_vs_golden(msm_golden):
+    _say("msm generator")
+    cases = msm_golden["scalar_mul_G"]
+    ks = np.array([c["k"] for c in cases], dtype=np.uint64)
+    pts = zmsm.generate_points(ks)
+    for p, c in zip(pts, cases):
+        assert oracle.affine_equal(p, oracle.affine_from_json(c["result"]))
+
+
+def test_precompute_vs_oracle():
+    """M2: twisted-Edwards image of each point vs the oracle's host precompute
+    ((y-x)/2, (y+x)/2, 4dt) of bls12_377_g1.h:606-618; ours is (y-x, y+x, 2dt)."""
+    _say("msm precompute")
+    rng = random.Random(12)
+    ks = _words([rng.randrange(1, R) for _ in range(200)], 4)
+    pts = oracle.mul_generator_many(ks)
+    with zmsm.MsmContext(pts) as ctx:
+        mine = ctx.precomputed(0, len(pts))
+    for i, p in enumerate(pts):
+        ok, ref = oracle.precompute_point(p)
+        assert ok
+        r = [oracle.from_mont(ref[6 * c:6 * c + 6]) for c in range(3)]
+        m = [oracle.from_mont(mine[i][6 * c:6 * c + 6]) for c in range(3)]
+        assert m[0] == 2 * r[0] % Q and m[1] == 2 * r[1] % Q and 2 * m[2] % Q == r[2]
+
+
+@pytest.mark.parametrize("use_void_abi", [False, True])
+def test_golden_cases(msm_golden, use_void_abi):
+    """Textbook big-int MSMs incl. the special points of msm_unit_tests.rs."""
+    for case in msm_golden["msm_cases"]:
+        _say(f"msm golden {case['name']}")
+        pts = np.array([oracle.affine_from_json(p) for p in case["points"]], dtype=zmsm.G1_AFFINE_DTYPE)
+        sc = np.array(case["scalars"], dtype=np.uint64)
+        with zmsm.MsmContext(pts) as ctx:
+            out = ctx.mult_abi(sc, 1) if use_void_abi else ctx.mult(sc, 1)
+        got = oracle.projective_to_affine(out[0])
+        assert oracle.affine_equal(got, oracle.affine_from_json(case["result"])), case["name"]
+        exp = oracle.msm(pts, sc, 1, threads=4)
+        assert out.tobytes() == exp.tobytes(), case["name"]
+
+
+def test_unrepresentable_points_at_chunk_boundaries(msm_golden):
+    """msm_unit_tests.rs:184-236: 4096 random points with special points at
+    1024, 2047, 3071, 4095, 4 batches."""
+    _say("msm chunk boundaries")
+    names = {c["name"]: c for c in msm_golden["msm_cases"]}
+    special = [oracle.affine_from_json(p) for p in names["unit_intermediate_infinity_1"]["points"]]
+    rng = random.Random(21)
+    n, batches = 4096, 4
+    ks = _words([rng.randrange(1, R) for _ in range(n)], 4)
+    pts = zmsm.generate_points(ks)
+    pts[1024] = special[0]
+    pts[2047] = special[2]
+    pts[3 * 1024 - 1] = special[1]
+    pts[4095] = special[2]
+    sc = _scalars(rng, n * batches)
+    with zmsm.MsmContext(pts) as ctx:
+        assert ctx.num_nonconvertible() == 4
+        out = ctx.mult(sc, batches)
+    exp = oracle.msm(pts, sc, batches, threads=8)
+    assert out.tobytes() == exp.tobytes()
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 5, 10, 12])
+def test_random_vs_oracle(log_n):
+    _say(f"msm random log_n={log_n}")
+    rng = random.Random(100 + log_n)
+    n = 1 << log_n
+    ks = _words([rng.randrange(1, R) for _ in range(n)], 4)
+    pts = zmsm.generate_points(ks)
+    sc = _scalars(rng, 2 * n)
+    with zmsm.MsmContext(pts) as ctx:
+        out = ctx.mult(sc, 2)
+    exp = oracle.msm(pts, sc, 2, threads=8)
+    assert out.tobytes() == exp.tobytes()
+
+
+@pytest.mark.parametrize("options", ["window=reference", "window=8", "window=13", "window=16", "window=19",
+                                     "window=12,log_u=0,log_g=0", "window=10,log_u=8,log_g=6"])
+def test_layouts_agree(options):
+    _say(f"msm layout {options}")
+    rng = random.Random(7)
+    n = 3000
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    s = [rng.randrange(R) for _ in range(n)]
+    with zmsm.MsmContext(pts, options) as ctx:
+        out = ctx.mult(_words(s, 4), 1)
+    e = sum(a * b for a, b in zip(s, ks)) % R
+    exp = oracle.g1_scalar_mul(oracle.generator(), e)
+    assert oracle.affine_equal(oracle.projective_to_affine(out[0]), exp)
+
+
+def test_edge_scalars():
+    _say("msm edge scalars")
+    rng = random.Random(31)
+    n = 257
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    g = oracle.generator()
+    with zmsm.MsmContext(pts) as ctx:
+        zero = ctx.mult(np.zeros((n, 4), dtype=np.uint64), 1)
+        assert not zero[0]["z"].any()  # infinity: z = 0
+        same = [R - 1] * n  # every bucket hit in the same place (hot bucket)
+        out = ctx.mult(_words(same, 4), 1)
+        e = sum((R - 1) * k for k in ks) % R
+        assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(g, e))
+        ones = ctx.mult(np.tile(np.array([1, 0, 0, 0], dtype=np.uint64), (n, 1)), 1)
+        e = sum(ks) % R
+        assert oracle.affine_equal(oracle.projective_to_affine(ones[0]), oracle.g1_scalar_mul(g, e))
+
+
+def test_device_scalars_and_prefix():
+    _say("msm device scalars + prefix")
+    rng = random.Random(44)
+    n = 5000
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    s = [rng.randrange(R) for _ in range(2 * n)]
+    ds = zmsm.DeviceBuffer(32 * 2 * n).upload(_words(s, 4))
+    g = oracle.generator()
+    with zmsm.MsmContext(pts) as ctx:
+        out = ctx.mult_device(ds.ptr, 2)
+        for b in range(2):
+            e = sum(a * k for a, k in zip(s[b * n:(b + 1) * n], ks)) % R
+            assert oracle.affine_equal(oracle.projective_to_affine(out[b]), oracle.g1_scalar_mul(g, e))
+        for m in (1, 100, 4097):
+            p = ctx.mult_prefix_device(m, ds.ptr)
+            e = sum(a * k for a, k in zip(s[:m], ks[:m])) % R
+            assert oracle.affine_equal(oracle.projective_to_affine(p), oracle.g1_scalar_mul(g, e))
+    ds.free()
+
+
+def test_large_closed_form():
+    """2^20 points generated on the GPU; result vs (sum s_i k_i mod r) G."""
+    _say("msm large 2^20")
+    rng = np.random.default_rng(2024)
+    n = 1 << 20
+    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    ks[:, 3] &= np.uint64((1 << 60) - 1)  # < 2^252 < r
+    sc = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    sc[:, 3] &= np.uint64((1 << 60) - 1)
+    dk = zmsm.DeviceBuffer(32 * n).upload(ks)
+    dp = zmsm.DeviceBuffer(104 * n)
+    zmsm.generate_points_device(dk.ptr, dp.ptr, n)
+    with zmsm.MsmContext((dp.ptr, n)) as ctx:
+        out = ctx.mult(sc, 1)
+    kk = [oracle.words_to_int(k) for k in ks]
+    ss = [oracle.words_to_int(s) for s in sc]
+    e = sum(a * b for a, b in zip(ss, kk)) % R
+    assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(oracle.generator(), e))
+    dk.free()
+    dp.free()

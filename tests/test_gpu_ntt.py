@@ -1,0 +1,115 @@
+"""Goldilocks NTT on the GPU through the C ABI, bit-exact against the oracle."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from hardcaml_zprize_amd import ntt as zntt
+from hardcaml_zprize_amd._lib import call
+
+pytestmark = pytest.mark.gpu
+P = oracle.P
+
+
+def _rand(n, seed):
+    return np.random.default_rng(seed).integers(0, P, size=n, dtype=np.uint64)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_kats(ntt_golden):
+    _say("ntt kats")
+    for name in ("kat_8pt_linear", "kat_8pt_random"):
+        kat = ntt_golden[name]
+        with zntt.NttGpuDriver(3) as d:
+            assert list(d.simple_evaluate(kat["input"])) == kat["output"]
+    kat = ntt_golden["kat_16pt_4step"]
+    with zntt.NttGpuDriver(4) as d:
+        assert list(d.simple_evaluate(kat["input"])) == kat["output"]
+    with zntt.NttGpuDriver(12) as d:
+        out = d.simple_evaluate([1, 2])  # zero padded to 4096
+        assert list(out[:8]) == ntt_golden["kat_4096pt_linear_first8"]["output_first8"]
+
+
+@pytest.mark.parametrize("log_n", list(range(1, 21)))
+def test_random_vs_oracle(log_n):
+    _say(f"ntt random log_n={log_n}")
+    a = _rand(1 << log_n, 1000 + log_n)
+    with zntt.NttGpuDriver(log_n) as d:
+        fwd = d.simple_evaluate(a, zntt.FORWARD)
+        assert np.array_equal(fwd, oracle.ntt(a))
+        inv = d.simple_evaluate(a, zntt.INVERSE)
+        assert np.array_equal(inv, oracle.ntt(a, inverse=True))
+
+
+@pytest.mark.parametrize("log_n", [21, 22, 23, 24])
+def test_large_vs_oracle(log_n):
+    # 2^24 as a 4096 x 4096 4-step, seed 1, bit-exact vs the CPU reference
+    _say(f"ntt large log_n={log_n}")
+    a = _rand(1 << log_n, 1 if log_n == 24 else log_n)
+    with zntt.NttGpuDriver(log_n) as d:
+        fwd = d.simple_evaluate(a, zntt.FORWARD)
+        assert np.array_equal(fwd, oracle.ntt(a))
+        back = d.simple_evaluate(fwd, zntt.INVERSE)
+        assert np.array_equal(back, a)
+
+
+def test_edge_values():
+    _say("ntt edge values")
+    with zntt.NttGpuDriver(14) as d:
+        n = 1 << 14
+        for a in (np.zeros(n, dtype=np.uint64), np.full(n, P - 1, dtype=np.uint64), np.ones(n, dtype=np.uint64)):
+            assert np.array_equal(d.simple_evaluate(a), oracle.ntt(a))
+        ones = d.simple_evaluate(np.ones(n, dtype=np.uint64))
+        assert int(ones[0]) == n and not ones[1:].any()
+
+
+def test_buffer_api_and_errors():
+    _say("ntt buffer api")
+    with zntt.NttGpuDriver(13, num_buffers=8) as d:
+        assert d.max_num_buffers_in_flight() == 8
+        bufs = [d.request_buffer() for _ in range(8)]
+        assert all(b is not None for b in bufs)
+        assert d.request_buffer() is None  # driver.h:146-150: NULL when all in use
+        inputs = [_rand(1 << 13, 50 + i) for i in range(8)]
+        for b, a, i in zip(bufs, inputs, range(8)):
+            d.transfer_input_vector_to_user_buffer(b, a)
+            d.enqueue_evaluation_async(b, i % 2)
+        for b, a, i in zip(bufs, inputs, range(8)):
+            d.wait_for_result(b)
+            out = d.transfer_user_buffer_to_output_vector(b)
+            assert np.array_equal(out, oracle.ntt(a, inverse=bool(i % 2)))
+            d.free_buffer(b)
+        with pytest.raises(zntt.ZpError):
+            d.free_buffer(bufs[0])  # driver.cpp:160-170 double free
+        with pytest.raises(zntt.ZpError):
+            d.simple_evaluate(np.zeros((1 << 13) + 1, dtype=np.uint64))  # driver.cpp:289-297
+
+
+def test_batch_mixed_sizes():
+    _say("ntt batch")
+    rng = np.random.default_rng(4)
+    log_ns = [int(x) for x in rng.integers(10, 17, size=12)]
+    dirs = [int(x) for x in rng.integers(0, 2, size=12)]
+    inputs = [_rand(1 << k, 70 + i) for i, k in enumerate(log_ns)]
+    ptrs = []
+    for a in inputs:
+        p = ctypes.c_void_p()
+        call("zp_device_malloc", ctypes.byref(p), a.nbytes)
+        call("zp_memcpy_h2d", p, a.ctypes.data, a.nbytes)
+        ptrs.append(p.value)
+    try:
+        zntt.ntt_batch_device(ptrs, log_ns, dirs)
+        call("zp_device_synchronize")
+        for p, a, k, dr in zip(ptrs, inputs, log_ns, dirs):
+            out = np.empty_like(a)
+            call("zp_memcpy_d2h", out.ctypes.data, ctypes.c_void_p(p), a.nbytes)
+            assert np.array_equal(out, oracle.ntt(a, inverse=bool(dr))), (k, dr)
+    finally:
+        for p in ptrs:
+            call("zp_device_free", ctypes.c_void_p(p))
