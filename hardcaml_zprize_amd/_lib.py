@@ -31,6 +31,9 @@ SIGNATURES = {
     "zp_ntt_wait": (_i32, [_vp, _i32]),
     "zp_ntt_free_buffer": (_i32, [_vp, _i32]),
     "zp_ntt_batch_device": (_i32, [_vp, _vp, _vp, _sz, _vp]),
+    "zp_ntt_batch_create": (_i32, [_vp, _vp, _vp, _sz, ctypes.POINTER(_vp)]),
+    "zp_ntt_batch_launch": (_i32, [_vp, _i32, _vp]),
+    "zp_ntt_batch_destroy": (_i32, [_vp]),
     "msm_init": (_vp, [ctypes.c_char_p, ctypes.c_ssize_t, _vp, ctypes.c_ssize_t]),
     "msm_mult": (None, [_vp, _vp, _u64, _vp]),
     "msm_free": (None, [_vp]),
@@ -41,6 +44,9 @@ SIGNATURES = {
     "zp_msm_mult_prefix_device": (_i32, [_vp, _sz, _vp, _vp, _vp]),
     "zp_msm_num_points": (_sz, [_vp]),
     "zp_msm_num_nonconvertible": (_sz, [_vp]),
+    "zp_msm_workspace_create": (_i32, [ctypes.POINTER(_vp)]),
+    "zp_msm_workspace_destroy": (_i32, [_vp]),
+    "zp_msm_batch_device": (_i32, [_vp, _vp, _sz, ctypes.c_char_p, _sz, _vp]),
     "zp_g1_mul_generator_device": (_i32, [_vp, _vp, _sz, _vp]),
     "zp_debug_fq_mul": (_i32, [_vp, _vp, _vp, _sz]),
     "zp_debug_msm_digits": (_i32, [ctypes.c_char_p, _sz, _vp, _sz, _vp, ctypes.POINTER(_i32)]),
@@ -51,6 +57,14 @@ SIGNATURES = {
     "zp_memcpy_h2d": (_i32, [_vp, _vp, _sz]),
     "zp_memcpy_d2h": (_i32, [_vp, _vp, _sz]),
     "zp_device_synchronize": (_i32, []),
+    "zp_stream_create": (_i32, [ctypes.POINTER(_vp)]),
+    "zp_stream_destroy": (_i32, [_vp]),
+    "zp_stream_synchronize": (_i32, [_vp]),
+    "zp_memcpy_h2d_async": (_i32, [_vp, _vp, _sz, _vp]),
+    "zp_memcpy_d2h_async": (_i32, [_vp, _vp, _sz, _vp]),
+    "zp_host_alloc": (_i32, [ctypes.POINTER(_vp), _sz]),
+    "zp_host_free": (_i32, [_vp]),
+    "zp_device_memory_info": (_i32, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
 }
 
 _lib = None

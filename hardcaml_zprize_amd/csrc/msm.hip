@@ -10,7 +10,9 @@
 #include <rocprim/device/device_scan.hpp>
 #include <map>
 #include <mutex>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -28,10 +30,19 @@ static const FConsts &consts() {
 }
 
 // ------------------------------------------------------------ layouts --
+static constexpr int MSM_MAXW = 32;
+
+struct HostLayout {
+  int W = 0;
+  int width[MSM_MAXW] = {};
+  int off[MSM_MAXW] = {};
+  uint32_t nb[MSM_MAXW] = {};
+};
+
 // The reference's 20 windows: 13 of 13 bits then 7 of 12 bits, top window
 // unsigned (zprize/msm_pippenger/hardcaml/src/config_utils.ml:7-41).
-static MsmLayout layout_reference() {
-  MsmLayout L{};
+static HostLayout layout_reference() {
+  HostLayout L;
   L.W = 20;
   int off = 0;
   for (int w = 0; w < 20; w++) {
@@ -43,8 +54,8 @@ static MsmLayout layout_reference() {
 }
 
 // Uniform windows of c bits (top window takes what is left of the 253 bits).
-static MsmLayout layout_uniform(int c) {
-  MsmLayout L{};
+static HostLayout layout_uniform(int c) {
+  HostLayout L;
   L.W = (253 + c - 1) / c;
   for (int w = 0; w < L.W; w++) {
     L.off[w] = w * c;
@@ -53,17 +64,13 @@ static MsmLayout layout_uniform(int c) {
   return L;
 }
 
-static void layout_finish(MsmLayout &L) {
-  uint32_t base = 0;
+static void layout_finish(HostLayout &L) {
   for (int w = 0; w < L.W; w++) {
     if (w == L.W - 1)
       L.nb[w] = 1u << (253 - L.off[w]);  // unsigned top window: slice + carry
     else
       L.nb[w] = 1u << (L.width[w] - 1);
-    L.base[w] = base;
-    base += L.nb[w];
   }
-  L.K = base;
 }
 
 // Window width for n points (the size-class policy; "window=<c>" overrides).
@@ -76,10 +83,10 @@ static int auto_window(size_t n) {
 }
 
 struct MsmOptions {
-  int window = 0;      // 0 = auto
+  int window = 0;  // 0 = auto
   bool reference = false;
-  int log_u = -1;      // entries per work unit, -1 = auto
-  int log_g = -1;      // buckets per window group, -1 = auto
+  int log_u = -1;  // entries per work unit, -1 = auto
+  int log_g = -1;  // buckets per window group, -1 = auto
 };
 
 static MsmOptions parse_options(const char *s, size_t len) {
@@ -109,8 +116,15 @@ static MsmOptions parse_options(const char *s, size_t len) {
   return o;
 }
 
-static MsmLayout choose_layout(const MsmOptions &o, size_t n) {
-  MsmLayout L = o.reference ? layout_reference() : layout_uniform(o.window ? o.window : auto_window(n));
+static int validate_options(const MsmOptions &o, const char *who) {
+  ZP_CHECK(o.window == 0 || (o.window >= 8 && o.window <= 24), ZP_ERR_INVALID_ARG, "%s: window=%d not in [8, 24]",
+           who, o.window);
+  ZP_CHECK(o.log_u <= 12 && o.log_g <= 10, ZP_ERR_INVALID_ARG, "%s: log_u/log_g out of range", who);
+  return ZP_OK;
+}
+
+static HostLayout choose_layout(const MsmOptions &o, size_t n) {
+  HostLayout L = o.reference ? layout_reference() : layout_uniform(o.window ? o.window : auto_window(n));
   layout_finish(L);
   return L;
 }
@@ -295,30 +309,14 @@ static unsigned grid_for(uint64_t n, unsigned threads = MSM_THREADS, unsigned ca
 }
 
 // --------------------------------------------------------- workspace --
-struct Workspace {
-  size_t cap_K = 0, cap_entries = 0, cap_units = 0, cap_groups = 0, cap_scalars = 0;
-  uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
-  uint32_t *ubucket = nullptr, *vals = nullptr;
-  Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
-  uint64_t *scalars = nullptr;
-  void *scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
-
-  void release() {
-    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, partial, buckets, groups, winsum, scalars, scan_tmp};
-    for (void *p : ps)
-      if (p) (void)hipFree(p);
-    *this = Workspace();
-  }
-};
-
 template <typename T>
 static int grow(T *&p, size_t &cap, size_t need) {
   if (need <= cap && p) return ZP_OK;
   if (p) (void)hipFree(p);
   p = nullptr;
+  cap = 0;
   if (hipMalloc(&p, std::max<size_t>(need, 1) * sizeof(T)) != hipSuccess) {
-    cap = 0;
+    p = nullptr;
     set_last_error("MSM workspace: allocation of %zu bytes failed", need * sizeof(T));
     return ZP_ERR_OUT_OF_MEMORY;
   }
@@ -326,73 +324,31 @@ static int grow(T *&p, size_t &cap, size_t need) {
   return ZP_OK;
 }
 
-static int ensure_workspace(Workspace &ws, const MsmLayout &L, uint32_t log_u, uint32_t log_g, size_t n,
-                            size_t n_scalars, hipStream_t stream) {
-  const size_t K = L.K;
-  size_t capK = ws.cap_K;
-  if (K + 1 > ws.cap_K) {
-    size_t c;
-    c = 0;
-    ZP_TRY(grow(ws.cnt, c, K + 1));
-    c = 0;
-    ZP_TRY(grow(ws.off, c, K + 1));
-    c = 0;
-    ZP_TRY(grow(ws.cursor, c, K + 1));
-    c = 0;
-    ZP_TRY(grow(ws.nunits, c, K + 1));
-    c = 0;
-    ZP_TRY(grow(ws.ustart, c, K + 1));
-    c = 0;
-    ZP_TRY(grow(ws.buckets, c, K + 1));
-    capK = K + 1;
-    // scan temp storage for K + 1 elements
-    size_t bytes = 0;
-    ZP_HIP(rocprim::exclusive_scan(nullptr, bytes, ws.cnt, ws.off, 0u, K + 1, rocprim::plus<uint32_t>(), stream));
-    if (ws.scan_tmp) (void)hipFree(ws.scan_tmp);
-    ws.scan_tmp = nullptr;
-    ZP_HIP(hipMalloc(&ws.scan_tmp, std::max<size_t>(bytes, 16)));
-    ws.scan_tmp_bytes = bytes;
-  }
-  ws.cap_K = capK;
-  const size_t entries = n * (size_t)L.W;
-  ZP_TRY(grow(ws.vals, ws.cap_entries, entries));
-  const size_t units = (entries >> log_u) + K + 1;
-  {
-    size_t c = ws.cap_units;
-    ZP_TRY(grow(ws.ubucket, c, units));
-    size_t c2 = ws.cap_units;
-    ZP_TRY(grow(ws.partial, c2, units));
-    ws.cap_units = std::max(c, c2);
-  }
-  size_t groups = 0;
-  for (int w = 0; w < L.W; w++) groups += (L.nb[w] + (1u << log_g) - 1) >> log_g;
-  ZP_TRY(grow(ws.groups, ws.cap_groups, groups + 1));
-  if (!ws.winsum) ZP_HIP(hipMalloc(&ws.winsum, MSM_MAXW * sizeof(Xyzt)));
-  if (n_scalars) ZP_TRY(grow(ws.scalars, ws.cap_scalars, 4 * n_scalars));
-  return ZP_OK;
-}
-
-static uint32_t auto_log_u(size_t entries, int requested) {
-  if (requested >= 0) return (uint32_t)requested;
-  // aim for ~2^19 work units; units of 8 .. 256 entries
-  int lu = 0;
-  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 8) lu++;
-  return (uint32_t)std::max(lu, 3);
-}
-
-static uint32_t auto_log_g(const MsmLayout &L, int requested) {
-  if (requested >= 0) return (uint32_t)requested;
-  uint32_t nbmax = 0;
-  for (int w = 0; w < L.W; w++) nbmax = std::max(nbmax, L.nb[w]);
-  // ~4096+ groups in total, at most 64 buckets per group, at least 4
-  uint32_t lg = 2;
-  while (lg < 6 && ((uint64_t)L.K >> lg) > 16384) lg++;
-  return lg;
-}
-
 }  // namespace zp
 
 using namespace zp;
+
+// Per-batch device working memory; one workspace serves one batch at a time.
+struct zp_msm_workspace {
+  size_t cap_K = 0, cap_K2 = 0, cap_K3 = 0, cap_K4 = 0, cap_K5 = 0, cap_Kb = 0;
+  size_t cap_entries = 0, cap_units = 0, cap_units2 = 0, cap_groups = 0, cap_gs = 0, cap_win = 0, cap_wsum = 0;
+  size_t cap_jobs = 0, cap_pt0 = 0, cap_b0 = 0, cap_scan = 0;
+  uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
+  uint32_t *ubucket = nullptr, *vals = nullptr, *gstart = nullptr, *pt0 = nullptr, *b0 = nullptr;
+  Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
+  MsmJobDev *jobs = nullptr;
+  WinDev *wins = nullptr;
+  uint8_t *scan_tmp = nullptr;
+  std::vector<Xyzt> h_winsum;
+
+  void release() {
+    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0,
+                  partial, buckets, groups, winsum, jobs, wins, scan_tmp};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    *this = zp_msm_workspace();
+  }
+};
 
 struct NonConv {
   uint64_t idx;
@@ -406,16 +362,34 @@ struct zp_msm_context {
   PackedPre *d_pts = nullptr;
   uint8_t *d_conv = nullptr;
   std::vector<NonConv> nonconv;
-  Workspace ws;
-  std::vector<Xyzt> h_winsum;
+  zp_msm_workspace ws;
+  uint64_t *d_scalars = nullptr;  // staging for host scalars
+  size_t cap_scalars = 0;
 };
 
-// ------------------------------------------------------------ finalize --
+namespace zp {
+
+static uint32_t auto_log_u(size_t entries, int requested) {
+  if (requested >= 0) return (uint32_t)requested;
+  // aim for ~2^19 work units; units of 8 .. 256 entries
+  int lu = 0;
+  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 8) lu++;
+  return (uint32_t)std::max(lu, 3);
+}
+
+static uint32_t auto_log_g(uint64_t K, int requested) {
+  if (requested >= 0) return (uint32_t)requested;
+  // ~16K groups in total, 4 .. 64 buckets per group
+  uint32_t lg = 2;
+  while (lg < 6 && (K >> lg) > 16384) lg++;
+  return lg;
+}
+
 // Window combine (driver.cpp:340-347 as Horner), conversion to affine
 // Weierstrass (bls12_377_g1.h:427-439) and the non-convertible fallback
 // (driver.cpp:349-372), then the wire form (x R, y R, R) or z = 0.
-static void finalize(const MsmLayout &L, const Xyzt *win, const std::vector<NonConv> &nonconv,
-                     const biginteger256_t *scalars_host_batch, g1_projective_t *out) {
+static void finalize(const HostLayout &L, const Xyzt *win, const std::vector<NonConv> &nonconv, size_t n,
+                     const biginteger256_t *nc_scalars, g1_projective_t *out) {
   const FConsts &K = consts();
   Xyzt acc = win[L.W - 1];
   for (int w = L.W - 2; w >= 0; w--) {
@@ -436,8 +410,10 @@ static void finalize(const MsmLayout &L, const Xyzt *win, const std::vector<NonC
     res.Y = f_canonical(f_mul(v, K.inv_cb));
     res.Z = K.one;
   }
+  size_t c = 0;
   for (const NonConv &nc : nonconv) {
-    const biginteger256_t &k = scalars_host_batch[nc.idx];
+    if (nc.idx >= n) continue;
+    const biginteger256_t &k = nc_scalars[c++];
     Jac kp;
     kp.X = f_zero();
     kp.Y = K.one;
@@ -460,80 +436,232 @@ static void finalize(const MsmLayout &L, const Xyzt *win, const std::vector<NonC
   memcpy(out->z.data, wz.w, 48);
 }
 
-// ------------------------------------------------------------ pipeline --
-static int run_msm(zp_msm_context *ctx, size_t n, const uint64_t *d_scalars, const biginteger256_t *h_scalars,
-                   g1_projective_t *out, hipStream_t stream) {
-  Workspace &ws = ctx->ws;
-  const MsmLayout L = choose_layout(ctx->opts, n);
-  const size_t entries = n * (size_t)L.W;
-  const uint32_t log_u = auto_log_u(entries, ctx->opts.log_u);
-  const uint32_t log_g = auto_log_g(L, ctx->opts.log_g);
-  ZP_TRY(ensure_workspace(ws, L, log_u, log_g, n, 0, stream));
-  const FConsts &K = consts();
-  const uint32_t KB = L.K;
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(KB + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)KB + 1, 0u);
-  ZP_HIP(hipGetLastError());
-  if (n) {
-    hipLaunchKernelGGL(k_digits_count, dim3(grid_for(n)), dim3(MSM_THREADS), 0, stream, d_scalars, ctx->d_conv,
-                       (uint64_t)n, L, ws.cnt);
-    ZP_HIP(hipGetLastError());
+// Host description of one job of a batch.
+struct HostJob {
+  zp_msm_context *ctx;
+  size_t n;
+  const uint64_t *d_scalars;
+  const biginteger256_t *h_scalars;  // optional host copy (for the fallback points)
+  g1_projective_t *out;
+  int window = 0;  // per-job window override (autotuner), 0 = none
+};
+
+static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, const MsmOptions &bopts,
+                     hipStream_t stream) {
+  if (count == 0) return ZP_OK;
+  // ---- plan
+  std::vector<HostLayout> layouts(count);
+  std::vector<MsmJobDev> jd(count);
+  std::vector<WinDev> wins;
+  std::vector<uint32_t> pt0(count), b0(count), gstart;
+  uint64_t K = 0, entries = 0, points = 0;
+  for (size_t j = 0; j < count; j++) {
+    const HostJob &h = jobs[j];
+    MsmOptions o = h.ctx->opts;
+    if (bopts.window || bopts.reference) {
+      o.window = bopts.window;
+      o.reference = bopts.reference;
+    }
+    if (h.window) {
+      o.window = h.window;
+      o.reference = false;
+    }
+    layouts[j] = choose_layout(o, h.n);
+    const HostLayout &L = layouts[j];
+    MsmJobDev &d = jd[j];
+    d.pts = h.ctx->d_pts;
+    d.conv = h.ctx->d_conv;
+    d.scalars = h.d_scalars;
+    d.n = (uint32_t)h.n;
+    d.pt0 = (uint32_t)points;
+    d.win0 = (uint32_t)wins.size();
+    d.W = (uint32_t)L.W;
+    d.bucket0 = (uint32_t)K;
+    pt0[j] = d.pt0;
+    b0[j] = d.bucket0;
+    for (int w = 0; w < L.W; w++) {
+      WinDev wd;
+      wd.job = (uint32_t)j;
+      wd.off = (uint32_t)L.off[w];
+      wd.width = (uint32_t)L.width[w];
+      wd.nb = L.nb[w];
+      wd.base = (uint32_t)K;
+      wd.top = w == L.W - 1;
+      K += L.nb[w];
+      wins.push_back(wd);
+    }
+    points += h.n;
+    entries += (uint64_t)h.n * L.W;
   }
-  size_t tb = ws.scan_tmp_bytes;
-  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.cnt, ws.off, 0u, KB + 1, rocprim::plus<uint32_t>(), stream));
-  ZP_HIP(hipMemcpyAsync(ws.cursor, ws.off, (KB + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-  if (n) {
-    hipLaunchKernelGGL(k_digits_scatter, dim3(grid_for(n)), dim3(MSM_THREADS), 0, stream, d_scalars, ctx->d_conv,
-                       (uint64_t)n, L, ws.cursor, ws.vals);
-    ZP_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(KB + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, KB, log_u,
-                     ws.nunits);
-  ZP_HIP(hipGetLastError());
-  tb = ws.scan_tmp_bytes;
-  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, KB + 1, rocprim::plus<uint32_t>(),
-                                 stream));
-  hipLaunchKernelGGL(k_fill_units, dim3(grid_for(KB)), dim3(MSM_THREADS), 0, stream, ws.nunits, ws.ustart, KB,
-                     ws.ubucket);
-  ZP_HIP(hipGetLastError());
-  const size_t max_units = (entries >> log_u) + KB + 1;
-  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_units, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream,
-                     ctx->d_pts, ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, ws.ustart + KB, log_u, ws.partial, K);
-  ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(KB)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart,
-                     ws.nunits, KB, ws.buckets, K);
-  ZP_HIP(hipGetLastError());
-  GroupLayout G{};
-  G.log_g = log_g;
+  ZP_CHECK(points < (1ull << 31) && K < (1ull << 31) && entries < (1ull << 32), ZP_ERR_TOO_LARGE,
+           "MSM batch too large (%llu points, %llu buckets)", (unsigned long long)points, (unsigned long long)K);
+  const uint32_t Wt = (uint32_t)wins.size();
+  const uint32_t log_u = auto_log_u(entries, bopts.log_u >= 0 ? bopts.log_u : jobs[0].ctx->opts.log_u);
+  const uint32_t log_g = auto_log_g(K, bopts.log_g >= 0 ? bopts.log_g : jobs[0].ctx->opts.log_g);
   uint32_t gs = 0;
-  for (int w = 0; w < L.W; w++) {
-    G.gstart[w] = gs;
-    gs += (L.nb[w] + (1u << log_g) - 1) >> log_g;
+  for (uint32_t w = 0; w < Wt; w++) {
+    gstart.push_back(gs);
+    gs += (wins[w].nb + (1u << log_g) - 1) >> log_g;
   }
-  G.gstart[L.W] = gs;
-  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, L, G, ws.groups,
-                     K);
+  gstart.push_back(gs);
+  const uint64_t max_units = (entries >> log_u) + K + 1;
+  ZP_CHECK(max_units < (1ull << 32), ZP_ERR_TOO_LARGE, "MSM batch: too many work units");
+  // ---- workspace
+  ZP_TRY(grow(ws.cnt, ws.cap_K, K + 1));
+  ZP_TRY(grow(ws.off, ws.cap_K2, K + 1));
+  ZP_TRY(grow(ws.cursor, ws.cap_K3, K + 1));
+  ZP_TRY(grow(ws.nunits, ws.cap_K4, K + 1));
+  ZP_TRY(grow(ws.ustart, ws.cap_K5, K + 1));
+  ZP_TRY(grow(ws.buckets, ws.cap_Kb, K + 1));
+  ZP_TRY(grow(ws.vals, ws.cap_entries, std::max<uint64_t>(entries, 1)));
+  ZP_TRY(grow(ws.ubucket, ws.cap_units, max_units));
+  ZP_TRY(grow(ws.partial, ws.cap_units2, max_units));
+  ZP_TRY(grow(ws.groups, ws.cap_groups, (size_t)gs + 1));
+  ZP_TRY(grow(ws.gstart, ws.cap_gs, gstart.size()));
+  ZP_TRY(grow(ws.wins, ws.cap_win, wins.size()));
+  ZP_TRY(grow(ws.winsum, ws.cap_wsum, wins.size()));
+  ZP_TRY(grow(ws.jobs, ws.cap_jobs, count));
+  ZP_TRY(grow(ws.pt0, ws.cap_pt0, count));
+  ZP_TRY(grow(ws.b0, ws.cap_b0, count));
+  size_t scan_bytes = 0;
+  ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
+                                 stream));
+  ZP_TRY(grow(ws.scan_tmp, ws.cap_scan, std::max<size_t>(scan_bytes, 16)));
+  ZP_HIP(hipMemcpyAsync(ws.jobs, jd.data(), count * sizeof(MsmJobDev), hipMemcpyHostToDevice, stream));
+  ZP_HIP(hipMemcpyAsync(ws.wins, wins.data(), wins.size() * sizeof(WinDev), hipMemcpyHostToDevice, stream));
+  ZP_HIP(hipMemcpyAsync(ws.pt0, pt0.data(), count * 4, hipMemcpyHostToDevice, stream));
+  ZP_HIP(hipMemcpyAsync(ws.b0, b0.data(), count * 4, hipMemcpyHostToDevice, stream));
+  ZP_HIP(hipMemcpyAsync(ws.gstart, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice, stream));
+  BatchDev B;
+  B.jobs = ws.jobs;
+  B.job_pt0 = ws.pt0;
+  B.job_b0 = ws.b0;
+  B.wins = ws.wins;
+  B.J = (uint32_t)count;
+  B.total_points = (uint32_t)points;
+  B.K = (uint32_t)K;
+  const FConsts &KC = consts();
+  // ---- kernels
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);
   ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_window_sum, dim3(L.W), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups, G,
-                     ws.winsum, K);
+  if (points) {
+    hipLaunchKernelGGL(k_digits<0>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.cnt, nullptr, nullptr,
+                       nullptr);
+    ZP_HIP(hipGetLastError());
+  }
+  size_t tb = ws.cap_scan;
+  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
+                                 stream));
+  ZP_HIP(hipMemcpyAsync(ws.cursor, ws.off, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+  if (points) {
+    hipLaunchKernelGGL(k_digits<1>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, nullptr, ws.cursor,
+                       ws.vals, nullptr);
+    ZP_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint32_t)K,
+                     log_u, ws.nunits);
   ZP_HIP(hipGetLastError());
-  ctx->h_winsum.resize(L.W);
-  ZP_HIP(hipMemcpyAsync(ctx->h_winsum.data(), ws.winsum, L.W * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
+  tb = ws.cap_scan;
+  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, (size_t)K + 1,
+                                 rocprim::plus<uint32_t>(), stream));
+  hipLaunchKernelGGL(k_fill_units, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.nunits, ws.ustart,
+                     (uint32_t)K, ws.ubucket);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_units, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream, B,
+                     ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, ws.ustart + K, log_u, ws.partial, KC);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.nunits,
+                     (uint32_t)K, ws.buckets, KC);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, ws.wins,
+                     ws.gstart, Wt, log_g, ws.groups, KC);
+  ZP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_window_sum, dim3(Wt), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups,
+                     ws.gstart, ws.winsum, KC);
+  ZP_HIP(hipGetLastError());
+  ws.h_winsum.resize(Wt);
+  ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
   ZP_HIP(hipStreamSynchronize(stream));
-  // Scalars of the non-convertible points (host copy when only device scalars exist).
-  std::vector<biginteger256_t> nc_scalars;
-  const biginteger256_t *hs = h_scalars;
-  std::vector<NonConv> nc_in_prefix;
-  for (const NonConv &nc : ctx->nonconv)
-    if (nc.idx < n) nc_in_prefix.push_back(nc);
-  if (!nc_in_prefix.empty() && !hs) {
-    nc_scalars.assign(n, biginteger256_t{{0, 0, 0, 0}});
-    for (const NonConv &nc : nc_in_prefix)
-      ZP_HIP(hipMemcpy(&nc_scalars[nc.idx], d_scalars + 4 * nc.idx, 32, hipMemcpyDeviceToHost));
-    hs = nc_scalars.data();
+  // ---- host finalisation (window combine, conversion, fallback points)
+  std::vector<int> status(count, ZP_OK);
+  auto fin = [&](size_t j) {
+    const HostJob &h = jobs[j];
+    std::vector<biginteger256_t> ncs;
+    for (const NonConv &nc : h.ctx->nonconv) {
+      if (nc.idx >= h.n) continue;
+      biginteger256_t k{{0, 0, 0, 0}};
+      if (h.h_scalars) {
+        k = h.h_scalars[nc.idx];
+      } else if (hipMemcpy(&k, h.d_scalars + 4 * nc.idx, 32, hipMemcpyDeviceToHost) != hipSuccess) {
+        status[j] = ZP_ERR_HIP;
+      }
+      ncs.push_back(k);
+    }
+    finalize(layouts[j], ws.h_winsum.data() + jd[j].win0, h.ctx->nonconv, h.n, ncs.data(), h.out);
+  };
+  const size_t nthreads = std::min<size_t>(count, 16);
+  if (nthreads <= 1) {
+    for (size_t j = 0; j < count; j++) fin(j);
+  } else {
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    for (size_t t = 0; t < nthreads; t++)
+      th.emplace_back([&] {
+        for (size_t j = next++; j < count; j = next++) fin(j);
+      });
+    for (auto &t : th) t.join();
   }
-  finalize(L, ctx->h_winsum.data(), nc_in_prefix, hs, out);
+  for (int st : status) {
+    ZP_CHECK(st == ZP_OK, st, "MSM batch: scalar copy for a fallback point failed");
+  }
   return ZP_OK;
 }
+
+// Digit dump for one job (diagnostics).
+static int dump_digits(const MsmOptions &o, const uint64_t *d_scalars, size_t n, int32_t *d_out, hipStream_t s,
+                       int *W_out) {
+  HostLayout L = choose_layout(o, n);
+  *W_out = L.W;
+  if (!d_out || !n) return ZP_OK;
+  MsmJobDev jd{};
+  jd.scalars = d_scalars;
+  jd.n = (uint32_t)n;
+  jd.W = (uint32_t)L.W;
+  std::vector<WinDev> wins(L.W);
+  uint32_t base = 0;
+  for (int w = 0; w < L.W; w++) {
+    wins[w] = WinDev{0, (uint32_t)L.off[w], (uint32_t)L.width[w], L.nb[w], base, (uint32_t)(w == L.W - 1)};
+    base += L.nb[w];
+  }
+  MsmJobDev *d_jobs = nullptr;
+  WinDev *d_wins = nullptr;
+  uint32_t *d_zero = nullptr;
+  ZP_HIP(hipMalloc(&d_jobs, sizeof(jd)));
+  ZP_HIP(hipMalloc(&d_wins, wins.size() * sizeof(WinDev)));
+  ZP_HIP(hipMalloc(&d_zero, 4));
+  ZP_HIP(hipMemcpy(d_jobs, &jd, sizeof(jd), hipMemcpyHostToDevice));
+  ZP_HIP(hipMemcpy(d_wins, wins.data(), wins.size() * sizeof(WinDev), hipMemcpyHostToDevice));
+  ZP_HIP(hipMemset(d_zero, 0, 4));
+  BatchDev B{};
+  B.jobs = d_jobs;
+  B.job_pt0 = d_zero;
+  B.job_b0 = d_zero;
+  B.wins = d_wins;
+  B.J = 1;
+  B.total_points = (uint32_t)n;
+  B.K = base;
+  hipLaunchKernelGGL(k_digits<2>, dim3(grid_for(n)), dim3(MSM_THREADS), 0, s, B, nullptr, nullptr, nullptr, d_out);
+  ZP_HIP(hipGetLastError());
+  ZP_HIP(hipStreamSynchronize(s));
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_wins);
+  (void)hipFree(d_zero);
+  return ZP_OK;
+}
+
+}  // namespace zp
+
+using namespace zp;
 
 // ---------------------------------------------------------------- init --
 static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_device) {
@@ -624,9 +752,7 @@ static int msm_init_common(const char *options, size_t options_len, const g1_aff
     return ZP_ERR_NO_DEVICE;
   }
   const MsmOptions opts = parse_options(options, options_len);
-  ZP_CHECK(opts.window == 0 || (opts.window >= 8 && opts.window <= 24), ZP_ERR_INVALID_ARG,
-           "msm_init: window=%d not in [8, 24]", opts.window);
-  ZP_CHECK(opts.log_u <= 12 && opts.log_g <= 10, ZP_ERR_INVALID_ARG, "msm_init: log_u/log_g out of range");
+  ZP_TRY(validate_options(opts, "msm_init"));
   zp_msm_context *ctx = new zp_msm_context();
   ctx->n = npoints;
   ctx->opts = opts;
@@ -672,6 +798,7 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->ws.release();
+  if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
   if (ctx->d_pts) (void)hipFree(ctx->d_pts);
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -683,13 +810,12 @@ extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t n
   ZP_CHECK(ctx != nullptr && out != nullptr && (scalars != nullptr || num_batches == 0 || ctx->n == 0),
            ZP_ERR_INVALID_ARG, "zp_msm_mult: NULL argument");
   const size_t n = ctx->n;
-  size_t cap = ctx->ws.cap_scalars;
-  ZP_TRY(grow(ctx->ws.scalars, cap, std::max<size_t>(4 * n, 4)));
-  ctx->ws.cap_scalars = cap;
+  ZP_TRY(grow(ctx->d_scalars, ctx->cap_scalars, std::max<size_t>(4 * n, 4)));
   for (uint64_t b = 0; b < num_batches; b++) {
     const biginteger256_t *hs = scalars + b * n;
-    if (n) ZP_HIP(hipMemcpyAsync(ctx->ws.scalars, hs, n * 32, hipMemcpyHostToDevice, ctx->stream));
-    ZP_TRY(run_msm(ctx, n, ctx->ws.scalars, hs, out + b, ctx->stream));
+    if (n) ZP_HIP(hipMemcpyAsync(ctx->d_scalars, hs, n * 32, hipMemcpyHostToDevice, ctx->stream));
+    HostJob job{ctx, n, ctx->d_scalars, hs, out + b};
+    ZP_TRY(run_batch(ctx->ws, &job, 1, MsmOptions(), ctx->stream));
   }
   return ZP_OK;
 }
@@ -705,8 +831,10 @@ extern "C" int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uin
   ZP_CHECK(ctx != nullptr && out != nullptr && (d_scalars != nullptr || num_batches == 0 || ctx->n == 0),
            ZP_ERR_INVALID_ARG, "zp_msm_mult_device: NULL argument");
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  for (uint64_t b = 0; b < num_batches; b++)
-    ZP_TRY(run_msm(ctx, ctx->n, reinterpret_cast<const uint64_t *>(d_scalars + b * ctx->n), nullptr, out + b, s));
+  for (uint64_t b = 0; b < num_batches; b++) {
+    HostJob job{ctx, ctx->n, reinterpret_cast<const uint64_t *>(d_scalars + b * ctx->n), nullptr, out + b};
+    ZP_TRY(run_batch(ctx->ws, &job, 1, MsmOptions(), s));
+  }
   return ZP_OK;
 }
 
@@ -716,7 +844,41 @@ extern "C" int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_proje
   ZP_CHECK(n <= ctx->n, ZP_ERR_TOO_LARGE, "zp_msm_mult_prefix_device: n %zu > %zu points", n, ctx->n);
   ZP_CHECK(d_scalars != nullptr || n == 0, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_device: NULL scalars");
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  return run_msm(ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), nullptr, out, s);
+  HostJob job{ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), nullptr, out};
+  return run_batch(ctx->ws, &job, 1, MsmOptions(), s);
+}
+
+// ------------------------------------------------------------- batched --
+extern "C" int zp_msm_workspace_create(zp_msm_workspace **out) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "zp_msm_workspace_create: NULL out");
+  *out = new zp_msm_workspace();
+  return ZP_OK;
+}
+
+extern "C" int zp_msm_workspace_destroy(zp_msm_workspace *ws) {
+  if (ws) {
+    ws->release();
+    delete ws;
+  }
+  return ZP_OK;
+}
+
+extern "C" int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs, size_t count, const char *options,
+                                   size_t options_len, void *stream) {
+  ZP_CHECK(ws != nullptr && (jobs != nullptr || count == 0), ZP_ERR_INVALID_ARG, "zp_msm_batch_device: NULL argument");
+  const MsmOptions o = parse_options(options, options_len);
+  ZP_TRY(validate_options(o, "zp_msm_batch_device"));
+  std::vector<HostJob> hj(count);
+  for (size_t j = 0; j < count; j++) {
+    const zp_msm_job &J = jobs[j];
+    ZP_CHECK(J.ctx != nullptr && J.out != nullptr, ZP_ERR_INVALID_ARG, "job %zu: NULL context or output", j);
+    ZP_CHECK(J.n <= J.ctx->n, ZP_ERR_TOO_LARGE, "job %zu: n %zu > %zu points", j, J.n, J.ctx->n);
+    ZP_CHECK(J.d_scalars != nullptr || J.n == 0, ZP_ERR_INVALID_ARG, "job %zu: NULL scalars", j);
+    ZP_CHECK(J.window == 0 || (J.window >= 8 && J.window <= 24), ZP_ERR_INVALID_ARG, "job %zu: window %d", j,
+             J.window);
+    hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), nullptr, J.out, J.window};
+  }
+  return run_batch(*ws, hj.data(), count, o, (hipStream_t)stream);
 }
 
 extern "C" size_t zp_msm_num_points(const zp_msm_context *ctx) { return ctx ? ctx->n : 0; }
@@ -845,20 +1007,19 @@ extern "C" int zp_debug_msm_digits(const char *options, size_t options_len, cons
                                    int32_t *digits, int *num_windows) {
   ZP_CHECK(num_windows != nullptr, ZP_ERR_INVALID_ARG, "zp_debug_msm_digits: NULL num_windows");
   const MsmOptions opts = parse_options(options, options_len);
-  ZP_CHECK(opts.window == 0 || (opts.window >= 8 && opts.window <= 24), ZP_ERR_INVALID_ARG,
-           "zp_debug_msm_digits: window=%d not in [8, 24]", opts.window);
-  const MsmLayout L = choose_layout(opts, n);
-  *num_windows = L.W;
+  ZP_TRY(validate_options(opts, "zp_debug_msm_digits"));
+  int W = 0;
+  ZP_TRY(dump_digits(opts, nullptr, n, nullptr, 0, &W));
+  *num_windows = W;
   if (!digits || !n) return ZP_OK;
   ZP_CHECK(scalars != nullptr, ZP_ERR_INVALID_ARG, "zp_debug_msm_digits: NULL scalars");
   uint64_t *ds = nullptr;
   int32_t *dd = nullptr;
   ZP_HIP(hipMalloc(&ds, n * 32));
-  ZP_HIP(hipMalloc(&dd, n * L.W * sizeof(int32_t)));
+  ZP_HIP(hipMalloc(&dd, n * W * sizeof(int32_t)));
   ZP_HIP(hipMemcpy(ds, scalars, n * 32, hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_digits_dump, dim3(grid_for(n)), dim3(MSM_THREADS), 0, 0, ds, (uint64_t)n, L, dd);
-  ZP_HIP(hipGetLastError());
-  ZP_HIP(hipMemcpy(digits, dd, n * L.W * sizeof(int32_t), hipMemcpyDeviceToHost));
+  ZP_TRY(dump_digits(opts, ds, n, dd, 0, &W));
+  ZP_HIP(hipMemcpy(digits, dd, n * W * sizeof(int32_t), hipMemcpyDeviceToHost));
   ZP_HIP(hipFree(ds));
   ZP_HIP(hipFree(dd));
   return ZP_OK;

@@ -1,10 +1,11 @@
-// MSM device kernels.
+// MSM device kernels.  One launch sequence serves a batch of jobs (one job =
+// one MSM over a prefix of a resident point table, with its own scalars and
+// window layout); the single msm_mult is a batch of one.
 //
-// Per msm_mult batch (N points, W windows):
 //   k_digits_count   signed-digit window decomposition of every scalar
 //                    (zprize/msm_pippenger/hardcaml/test/top/utils.ml:243-288,
 //                    generalised to any window widths) and a histogram of
-//                    (window, bucket) keys; zero digits are skipped
+//                    global bucket keys; zero digits are skipped
 //                    (libs/pippenger/src/controller.ml:128-132);
 //   scan             bucket offsets and work-unit offsets;
 //   k_digits_scatter counting-sort placement: vals[pos] = point | sign << 31;
@@ -18,7 +19,7 @@
 //                    sum_b (b - lo + 1) S_b + lo * sum_b S_b;
 //   k_window_sum     one workgroup per window: tree sum of the group results.
 // The window combine and the affine conversion run on the host (a few
-// thousand field operations, the part of driver.cpp:340-375 left).
+// thousand field operations per job, the part of driver.cpp:340-375 left).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,47 +28,63 @@
 
 namespace zp {
 
-static constexpr int MSM_MAXW = 32;
 static constexpr int MSM_THREADS = 256;
 
-struct MsmLayout {
-  int W;
-  int width[MSM_MAXW];
-  int off[MSM_MAXW];
-  uint32_t nb[MSM_MAXW];    // buckets of window w
-  uint32_t base[MSM_MAXW];  // first global bucket index of window w
-  uint32_t K;               // total buckets
+struct MsmJobDev {
+  const PackedPre *pts;     // resident table of this job
+  const uint8_t *conv;      // 0 = point without a twisted-Edwards image
+  const uint64_t *scalars;  // n scalars, 4 words each
+  uint32_t n;               // points used (a prefix of the table)
+  uint32_t pt0;             // first global point slot of this job
+  uint32_t win0, W;         // windows [win0, win0 + W)
+  uint32_t bucket0;         // first global bucket of this job
 };
+
+struct WinDev {
+  uint32_t job;
+  uint32_t off, width;  // bit offset and width of the window
+  uint32_t nb, base;    // buckets, first global bucket
+  uint32_t top;         // 1 = unsigned top window (slice + carry)
+};
+
+// Largest i in [0, n) with key[i] <= v (keys ascending, key[0] == 0).
+__device__ __forceinline__ uint32_t upper_index(const uint32_t *key, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (key[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 // Bits [off, off + len) of a 253-bit scalar (bits >= 253 read as 0,
 // config.ml:5-9 scalar_bits = 253).
 __device__ __forceinline__ uint32_t scalar_slice(const uint64_t s[4], int off, int len) {
-  uint32_t v = 0;
   const int w = off >> 6, b = off & 63;
   uint64_t lo = s[w] >> b;
   if (b && w + 1 < 4) lo |= s[w + 1] << (64 - b);
-  v = (uint32_t)lo;
-  int top = off + len;
-  if (top > 253) len = 253 - off;
+  if (off + len > 253) len = 253 - off;
   if (len <= 0) return 0;
-  return v & ((len >= 32) ? 0xffffffffu : ((1u << len) - 1));
+  return (uint32_t)lo & ((len >= 32) ? 0xffffffffu : ((1u << len) - 1));
 }
 
-// Digit of window w given the incoming carry; returns signed digit.
-__device__ __forceinline__ int32_t window_digit(const uint64_t s[4], const MsmLayout &L, int w, uint32_t &carry) {
-  const int width = L.width[w];
-  const uint32_t slice = scalar_slice(s, L.off[w], width) + carry;
-  if (w == L.W - 1) {
+// Signed digit of one window given the incoming carry.
+__device__ __forceinline__ int32_t window_digit(const uint64_t s[4], const WinDev &wd, uint32_t &carry) {
+  const uint32_t slice = scalar_slice(s, wd.off, wd.width) + carry;
+  if (wd.top) {
     carry = 0;
     return (int32_t)slice;
   }
-  if (slice >= (1u << width)) {
+  if (slice >= (1u << wd.width)) {
     carry = 1;
     return 0;
   }
-  if (slice >= (1u << (width - 1))) {
+  if (slice >= (1u << (wd.width - 1))) {
     carry = 1;
-    return (int32_t)slice - (int32_t)(1u << width);
+    return (int32_t)slice - (int32_t)(1u << wd.width);
   }
   carry = 0;
   return (int32_t)slice;
@@ -75,64 +92,58 @@ __device__ __forceinline__ int32_t window_digit(const uint64_t s[4], const MsmLa
 
 __device__ __forceinline__ void load_scalar(const uint64_t *scalars, uint64_t i, uint64_t s[4]) {
   const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(scalars + 4 * i);
-  ulonglong2 a = p[0], b = p[1];
+  const ulonglong2 a = p[0], b = p[1];
   s[0] = a.x;
   s[1] = a.y;
   s[2] = b.x;
   s[3] = b.y;
 }
 
-__global__ void __launch_bounds__(MSM_THREADS) k_digits_count(const uint64_t *scalars, const uint8_t *conv, uint64_t n,
-                                                              MsmLayout L, uint32_t *cnt) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    if (!conv[i]) continue;
+struct BatchDev {
+  const MsmJobDev *jobs;
+  const uint32_t *job_pt0;  // J entries (ascending)
+  const uint32_t *job_b0;   // J entries: first bucket per job (ascending)
+  const WinDev *wins;
+  uint32_t J;
+  uint32_t total_points;
+  uint32_t K;  // total buckets
+};
+
+// MODE 0: histogram; MODE 1: scatter; MODE 2: dump digits (diagnostics).
+template <int MODE>
+__global__ void __launch_bounds__(MSM_THREADS) k_digits(BatchDev B, uint32_t *cnt, uint32_t *cursor, uint32_t *vals,
+                                                        int32_t *dump) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < B.total_points; g += gridDim.x * blockDim.x) {
+    const uint32_t j = upper_index(B.job_pt0, B.J, g);
+    const MsmJobDev jb = B.jobs[j];
+    const uint32_t i = g - jb.pt0;
+    if (MODE != 2 && !jb.conv[i]) continue;  // scalar zeroed, driver.cpp:398-406
     uint64_t s[4];
-    load_scalar(scalars, i, s);
+    load_scalar(jb.scalars, i, s);
     uint32_t carry = 0;
-    for (int w = 0; w < L.W; w++) {
-      const int32_t d = window_digit(s, L, w, carry);
-      if (d != 0) {
-        const uint32_t mag = (uint32_t)(d < 0 ? -d : d);
-        atomicAdd(&cnt[L.base[w] + mag - 1], 1u);
+    for (uint32_t w = 0; w < jb.W; w++) {
+      const WinDev wd = B.wins[jb.win0 + w];
+      const int32_t d = window_digit(s, wd, carry);
+      if (MODE == 2) {
+        dump[(uint64_t)i * jb.W + w] = d;
+        continue;
+      }
+      if (d == 0) continue;
+      const uint32_t key = wd.base + (uint32_t)(d < 0 ? -d : d) - 1;
+      if (MODE == 0) {
+        atomicAdd(&cnt[key], 1u);
+      } else {
+        const uint32_t pos = atomicAdd(&cursor[key], 1u);
+        vals[pos] = i | (d < 0 ? 0x80000000u : 0u);
       }
     }
-  }
-}
-
-__global__ void __launch_bounds__(MSM_THREADS) k_digits_scatter(const uint64_t *scalars, const uint8_t *conv,
-                                                                uint64_t n, MsmLayout L, uint32_t *cursor,
-                                                                uint32_t *vals) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    if (!conv[i]) continue;
-    uint64_t s[4];
-    load_scalar(scalars, i, s);
-    uint32_t carry = 0;
-    for (int w = 0; w < L.W; w++) {
-      const int32_t d = window_digit(s, L, w, carry);
-      if (d != 0) {
-        const uint32_t mag = (uint32_t)(d < 0 ? -d : d);
-        const uint32_t pos = atomicAdd(&cursor[L.base[w] + mag - 1], 1u);
-        vals[pos] = (uint32_t)i | (d < 0 ? 0x80000000u : 0u);
-      }
-    }
-  }
-}
-
-// Signed digits of every scalar for inspection (diagnostics).
-__global__ void k_digits_dump(const uint64_t *scalars, uint64_t n, MsmLayout L, int32_t *out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t s[4];
-    load_scalar(scalars, i, s);
-    uint32_t carry = 0;
-    for (int w = 0; w < L.W; w++) out[i * L.W + w] = window_digit(s, L, w, carry);
   }
 }
 
 // nunits[k] = ceil(cnt[k] / U); k == K gets 0 (so the scan's last entry is the total).
 __global__ void k_units_per_bucket(const uint32_t *cnt, uint32_t K, uint32_t log_u, uint32_t *nunits) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= K; k += gridDim.x * blockDim.x) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= K; k += gridDim.x * blockDim.x)
     nunits[k] = k < K ? (cnt[k] + (1u << log_u) - 1) >> log_u : 0;
-  }
 }
 
 __global__ void k_fill_units(const uint32_t *nunits, const uint32_t *ustart, uint32_t K, uint32_t *ubucket) {
@@ -142,16 +153,14 @@ __global__ void k_fill_units(const uint32_t *nunits, const uint32_t *ustart, uin
   }
 }
 
-__device__ __forceinline__ void store_xyzt(Xyzt *dst, const Xyzt &v) { *dst = v; }
-
-__global__ void __launch_bounds__(MSM_THREADS) k_accumulate(const PackedPre *pts, const uint32_t *vals,
-                                                            const uint32_t *off, const uint32_t *cnt,
-                                                            const uint32_t *ustart, const uint32_t *ubucket,
-                                                            const uint32_t *total_units, uint32_t log_u,
-                                                            Xyzt *partial, FConsts K) {
+__global__ void __launch_bounds__(MSM_THREADS) k_accumulate(BatchDev B, const uint32_t *vals, const uint32_t *off,
+                                                            const uint32_t *cnt, const uint32_t *ustart,
+                                                            const uint32_t *ubucket, const uint32_t *total_units,
+                                                            uint32_t log_u, Xyzt *partial, FConsts K) {
   const uint32_t total = *total_units;
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
     const uint32_t k = ubucket[u];
+    const PackedPre *pts = B.jobs[upper_index(B.job_b0, B.J, k)].pts;
     const uint32_t s = u - ustart[k];
     const uint32_t b0 = off[k] + (s << log_u);
     const uint32_t b1 = min(off[k] + cnt[k], b0 + (1u << log_u));
@@ -161,7 +170,7 @@ __global__ void __launch_bounds__(MSM_THREADS) k_accumulate(const PackedPre *pts
       const PrePt p = pre_unpack(pts[v & 0x7fffffffu]);
       ec_madd(acc, p, (v >> 31) != 0, K);
     }
-    store_xyzt(&partial[u], acc);
+    partial[u] = acc;
   }
 }
 
@@ -189,25 +198,20 @@ __device__ __forceinline__ Xyzt ec_mul_small(const Xyzt &p, uint32_t m, const FC
   return r;
 }
 
-// Group j of window w covers buckets [lo, lo + g) (clipped to nb[w]).
-// gstart[w] = index of the first group of window w; groups[gstart[w] + j].
-struct GroupLayout {
-  uint32_t log_g;
-  uint32_t gstart[MSM_MAXW + 1];
-};
-
-__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *buckets, MsmLayout L, GroupLayout G,
-                                                               Xyzt *groups, FConsts K) {
-  const uint32_t total = G.gstart[L.W];
+// Group j of window w covers buckets [j g, j g + g) clipped to nb[w];
+// gstart[w] = first group of window w (W_total + 1 entries).
+__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *buckets, const WinDev *wins,
+                                                               const uint32_t *gstart, uint32_t W_total,
+                                                               uint32_t log_g, Xyzt *groups, FConsts K) {
+  const uint32_t total = gstart[W_total];
   for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < total; gi += gridDim.x * blockDim.x) {
-    int w = 0;
-    while (gi >= G.gstart[w + 1]) w++;
-    const uint32_t j = gi - G.gstart[w];
-    const uint32_t lo = j << G.log_g;
-    const uint32_t hi = min(L.nb[w], lo + (1u << G.log_g));
+    const uint32_t w = upper_index(gstart, W_total, gi);
+    const WinDev wd = wins[w];
+    const uint32_t lo = (gi - gstart[w]) << log_g;
+    const uint32_t hi = min(wd.nb, lo + (1u << log_g));
     Xyzt run = ec_identity(K), acc = ec_identity(K);
     for (int b = (int)hi - 1; b >= (int)lo; b--) {
-      ec_add(run, buckets[L.base[w] + b], K);
+      ec_add(run, buckets[wd.base + b], K);
       ec_add(acc, run, K);
     }
     // bucket b holds digit magnitude b + 1: weight (b + 1) = (b - lo + 1) + lo
@@ -220,10 +224,11 @@ __global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *bucke
 }
 
 // One workgroup per window: tree sum of its groups into out[w].
-__global__ void __launch_bounds__(MSM_THREADS) k_window_sum(const Xyzt *groups, GroupLayout G, Xyzt *out, FConsts K) {
+__global__ void __launch_bounds__(MSM_THREADS) k_window_sum(const Xyzt *groups, const uint32_t *gstart, Xyzt *out,
+                                                            FConsts K) {
   extern __shared__ Xyzt sh[];
-  const int w = blockIdx.x;
-  const uint32_t g0 = G.gstart[w], g1 = G.gstart[w + 1];
+  const uint32_t w = blockIdx.x;
+  const uint32_t g0 = gstart[w], g1 = gstart[w + 1];
   Xyzt acc = ec_identity(K);
   for (uint32_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) ec_add(acc, groups[g], K);
   sh[threadIdx.x] = acc;

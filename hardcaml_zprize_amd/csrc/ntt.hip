@@ -22,6 +22,7 @@
 
 #include "common.hpp"
 #include "gl64.hpp"
+#include "ntt4096.hpp"
 #include "ntt_kernels.hpp"
 
 namespace zp {
@@ -44,6 +45,7 @@ struct NttTables {
   uint64_t *d_sub[2] = {nullptr, nullptr};  // w_4096^i, i < 2048 (dir 0 / dir 1)
   uint64_t *d_lo[2] = {nullptr, nullptr};   // w_n^i, i < 4096
   uint64_t *d_hi[2] = {nullptr, nullptr};   // w_n^(4096 i), i < max(1, n / 4096)
+  uint64_t *d_4096[2] = {nullptr, nullptr}; // w_4096^i, i < 4096
   uint64_t n_inv = 1;
 };
 
@@ -64,7 +66,7 @@ static int make_tables(unsigned log_n, NttTables **out) {
   t->log_n = log_n;
   const uint64_t n = 1ull << log_n;
   const uint64_t n_hi = std::max<uint64_t>(1, n >> 12);
-  std::vector<uint64_t> sub(2048), lo(4096), hi(n_hi);
+  std::vector<uint64_t> sub(2048), lo(4096), hi(n_hi), full(4096);
   for (int dir = 0; dir < 2; dir++) {
     uint64_t w4096 = gl_root_host(12);
     uint64_t wn = gl_root_host(log_n);
@@ -73,13 +75,18 @@ static int make_tables(unsigned log_n, NttTables **out) {
       wn = gl_inv_host(wn);
     }
     uint64_t acc = 1;
-    for (int i = 0; i < 2048; i++, acc = gl_mul_host(acc, w4096)) sub[i] = acc;
+    for (int i = 0; i < 4096; i++, acc = gl_mul_host(acc, w4096)) {
+      full[i] = acc;
+      if (i < 2048) sub[i] = acc;
+    }
     acc = 1;
     for (int i = 0; i < 4096; i++, acc = gl_mul_host(acc, wn)) lo[i] = acc;
     uint64_t step = gl_pow_host(wn, 4096);
     acc = 1;
     for (uint64_t i = 0; i < n_hi; i++, acc = gl_mul_host(acc, step)) hi[i] = acc;
     ZP_HIP(hipMalloc(&t->d_sub[dir], 2048 * 8));
+    ZP_HIP(hipMalloc(&t->d_4096[dir], 4096 * 8));
+    ZP_HIP(hipMemcpy(t->d_4096[dir], full.data(), 4096 * 8, hipMemcpyHostToDevice));
     ZP_HIP(hipMalloc(&t->d_lo[dir], 4096 * 8));
     ZP_HIP(hipMalloc(&t->d_hi[dir], n_hi * 8));
     ZP_HIP(hipMemcpy(t->d_sub[dir], sub.data(), 2048 * 8, hipMemcpyHostToDevice));
@@ -110,10 +117,70 @@ static NttShape shape_of(unsigned log_n) {
   return s;
 }
 
+static int g_ntt_tile = -1;  // T of the 4096-point kernels (ZP_NTT_TILE=4|8)
+
+static int ntt_tile() {
+  if (g_ntt_tile < 0) {
+    const char *e = getenv("ZP_NTT_TILE");
+    int v = e ? atoi(e) : 8;
+    g_ntt_tile = (v == 4) ? 4 : 8;
+  }
+  return g_ntt_tile;
+}
+
+// Job tables for the batched form: data_tab[j] / scratch_tab[j] (device arrays), jobs = grid.y.
+struct JobTabs {
+  uint64_t *const *data = nullptr;
+  uint64_t *const *scratch = nullptr;
+  unsigned jobs = 1;
+};
+
+template <int T>
+static int launch_4096_pair(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, hipStream_t stream,
+                            const JobTabs &jt) {
+  const uint64_t n1 = 4096, n2 = 4096;
+  Ntt4096Args a{};
+  a.n1 = n1;
+  a.n2 = n2;
+  a.tw4096 = t->d_4096[dir];
+  a.tw_lo = t->d_lo[dir];
+  a.tw_hi = t->d_hi[dir];
+  a.tiles = (uint32_t)(n2 / T);
+  a.in = d_data;
+  a.out = d_scratch;
+  a.in_tab = jt.data;
+  a.out_tab = jt.scratch;
+  a.scale = 1;
+  const size_t lds = ntt4096_lds_bytes<T>();
+  const dim3 grid1(a.tiles, jt.jobs);
+  if (dir)
+    hipLaunchKernelGGL((k_ntt4096_cols<T, 1>), grid1, dim3(64 * T), lds, stream, a);
+  else
+    hipLaunchKernelGGL((k_ntt4096_cols<T, 0>), grid1, dim3(64 * T), lds, stream, a);
+  ZP_HIP(hipGetLastError());
+  a.in = d_scratch;
+  a.out = d_data;
+  a.in_tab = jt.scratch;
+  a.out_tab = jt.data;
+  a.scale = dir ? t->n_inv : 1;
+  a.tiles = (uint32_t)(n1 / T);
+  const dim3 grid2(a.tiles, jt.jobs);
+  if (dir)
+    hipLaunchKernelGGL((k_ntt4096_rows<T, 1>), grid2, dim3(64 * T), lds, stream, a);
+  else
+    hipLaunchKernelGGL((k_ntt4096_rows<T, 0>), grid2, dim3(64 * T), lds, stream, a);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
 // Enqueue one in-place transform.  scratch must hold n elements when log_n > 12.
 static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir,
-                            hipStream_t stream) {
+                            hipStream_t stream, const JobTabs &jt = JobTabs()) {
   const NttShape sh = shape_of(t->log_n);
+  if (sh.log_n1 == 12 && sh.log_n2 == 12) {
+    return ntt_tile() == 4 ? launch_4096_pair<4>(t, d_data, d_scratch, dir, stream, jt)
+                           : launch_4096_pair<8>(t, d_data, d_scratch, dir, stream, jt);
+  }
   const uint64_t n = 1ull << sh.log_n;
   const uint64_t scale = dir ? t->n_inv : 1;
   if (sh.log_n2 == 0) {
@@ -128,7 +195,9 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
     a.out_seq_stride = n;
     a.tw_sub = t->d_sub[dir];
     a.scale = scale;
-    return launch_dft(a, 1, stream);
+    a.in_tab = jt.data;
+    a.out_tab = jt.data;
+    return launch_dft(a, 1, stream, jt.jobs);
   }
   const uint64_t n1 = 1ull << sh.log_n1, n2 = 1ull << sh.log_n2;
   // Pass 1: columns (length n1, n2 of them) + twiddle, data -> scratch.
@@ -145,7 +214,9 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
   p1.tw_lo = t->d_lo[dir];
   p1.tw_hi = t->d_hi[dir];
   p1.scale = 1;
-  ZP_TRY(launch_dft(p1, n2 >> p1.log_T, stream));
+  p1.in_tab = jt.data;
+  p1.out_tab = jt.scratch;
+  ZP_TRY(launch_dft(p1, n2 >> p1.log_T, stream, jt.jobs));
   // Pass 2: rows (length n2, n1 of them), scratch -> data transposed.
   DftArgs p2{};
   p2.in = d_scratch;
@@ -158,7 +229,9 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
   p2.out_seq_stride = 1;
   p2.tw_sub = t->d_sub[dir];
   p2.scale = scale;
-  return launch_dft(p2, n1 >> p2.log_T, stream);
+  p2.in_tab = jt.scratch;
+  p2.out_tab = jt.data;
+  return launch_dft(p2, n1 >> p2.log_T, stream, jt.jobs);
 }
 
 }  // namespace zp
@@ -333,45 +406,137 @@ extern "C" int zp_ntt_free_buffer(zp_ntt_plan *p, int id) {
 }
 
 // ----------------------------------------------------------------- batch --
-static std::mutex g_batch_mu;
-static std::map<int, std::pair<uint64_t *, size_t>> g_batch_scratch;  // device -> arena
+// Jobs grouped by (log_n, dir): one launch sequence per group with the job
+// index on grid.y, per-job data / scratch pointers in device tables.  The
+// whole sequence is captured into a HIP graph at the first launch and
+// replayed afterwards.
+struct NttGroup {
+  unsigned log_n;
+  int dir;
+  unsigned jobs;
+  size_t tab_off;  // index of this group's first entry in the pointer tables
+};
 
-extern "C" int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
-                                   void *stream) {
-  ZP_CHECK(count == 0 || (d_ptrs && log_ns && dirs), ZP_ERR_INVALID_ARG, "zp_ntt_batch_device: NULL argument");
-  size_t scratch_need = 0;
+struct zp_ntt_batch {
+  std::vector<NttGroup> groups;
+  uint64_t *d_scratch = nullptr;  // arena
+  uint64_t **d_tabs = nullptr;    // [data tables | scratch tables]
+  size_t count = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int batch_launch_eager(zp_ntt_batch *b, hipStream_t stream) {
+  for (const NttGroup &g : b->groups) {
+    NttTables *t = nullptr;
+    ZP_TRY(make_tables(g.log_n, &t));
+    JobTabs jt;
+    jt.data = b->d_tabs + g.tab_off;
+    jt.scratch = b->d_tabs + b->count + g.tab_off;
+    jt.jobs = g.jobs;
+    ZP_TRY(launch_transform(t, nullptr, nullptr, g.dir, stream, jt));
+  }
+  return ZP_OK;
+}
+
+extern "C" int zp_ntt_batch_create(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
+                                   zp_ntt_batch **out) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "zp_ntt_batch_create: out is NULL");
+  *out = nullptr;
+  ZP_CHECK(count == 0 || (d_ptrs && log_ns && dirs), ZP_ERR_INVALID_ARG, "zp_ntt_batch_create: NULL argument");
   for (size_t i = 0; i < count; i++) {
     ZP_CHECK(log_ns[i] >= 1 && log_ns[i] <= 24, ZP_ERR_INVALID_ARG, "job %zu: log_n %u not in [1, 24]", i, log_ns[i]);
     ZP_CHECK(dirs[i] == 0 || dirs[i] == 1, ZP_ERR_INVALID_ARG, "job %zu: dir %d", i, dirs[i]);
     ZP_CHECK(d_ptrs[i] != nullptr, ZP_ERR_INVALID_ARG, "job %zu: NULL data", i);
-    if (log_ns[i] > 12) scratch_need += 1ull << log_ns[i];
   }
-  int dev = 0;
-  ZP_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lk(g_batch_mu);
-  auto &arena = g_batch_scratch[dev];
-  if (arena.second < scratch_need) {
-    if (arena.first) {
-      ZP_HIP(hipStreamSynchronize((hipStream_t)stream));
-      ZP_HIP(hipFree(arena.first));
-      arena.first = nullptr;
-      arena.second = 0;
-    }
-    ZP_HIP(hipMalloc(&arena.first, scratch_need * 8));
-    arena.second = scratch_need;
+  // order jobs by (log_n, dir)
+  std::vector<size_t> order(count);
+  for (size_t i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return log_ns[x] != log_ns[y] ? log_ns[x] < log_ns[y] : dirs[x] < dirs[y];
+  });
+  zp_ntt_batch *b = new zp_ntt_batch();
+  b->count = count;
+  std::vector<uint64_t *> tabs(2 * count + 1, nullptr);
+  size_t scratch_need = 0;
+  for (size_t i = 0; i < count; i++) {
+    const size_t j = order[i];
+    if (b->groups.empty() || b->groups.back().log_n != log_ns[j] || b->groups.back().dir != dirs[j])
+      b->groups.push_back(NttGroup{log_ns[j], dirs[j], 0, i});
+    b->groups.back().jobs++;
+    if (log_ns[j] > 12) scratch_need += 1ull << log_ns[j];
   }
+  int st = ZP_OK;
+  if (scratch_need && hipMalloc(&b->d_scratch, scratch_need * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
   size_t off = 0;
   for (size_t i = 0; i < count; i++) {
-    NttTables *t = nullptr;
-    ZP_TRY(make_tables(log_ns[i], &t));
-    uint64_t *scr = nullptr;
-    if (log_ns[i] > 12) {
-      scr = arena.first + off;
-      off += 1ull << log_ns[i];
+    const size_t j = order[i];
+    tabs[i] = d_ptrs[j];
+    if (log_ns[j] > 12 && b->d_scratch) {
+      tabs[count + i] = b->d_scratch + off;
+      off += 1ull << log_ns[j];
     }
-    ZP_TRY(launch_transform(t, d_ptrs[i], scr, dirs[i], (hipStream_t)stream));
   }
+  if (st == ZP_OK && hipMalloc(&b->d_tabs, tabs.size() * sizeof(uint64_t *)) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
+  if (st == ZP_OK &&
+      hipMemcpy(b->d_tabs, tabs.data(), tabs.size() * sizeof(uint64_t *), hipMemcpyHostToDevice) != hipSuccess)
+    st = ZP_ERR_HIP;
+  for (const NttGroup &g : b->groups) {
+    NttTables *t = nullptr;
+    if (st == ZP_OK) st = make_tables(g.log_n, &t);
+  }
+  if (st != ZP_OK) {
+    if (st == ZP_ERR_OUT_OF_MEMORY) set_last_error("zp_ntt_batch_create: device allocation failed");
+    zp_ntt_batch_destroy(b);
+    return st;
+  }
+  *out = b;
   return ZP_OK;
+}
+
+extern "C" int zp_ntt_batch_launch(zp_ntt_batch *b, int use_graph, void *stream) {
+  ZP_CHECK(b != nullptr, ZP_ERR_INVALID_ARG, "zp_ntt_batch_launch: NULL batch");
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) return batch_launch_eager(b, s);
+  if (!b->exec) {
+    // Capture on a private stream (the null stream cannot be captured).
+    hipStream_t cap = nullptr;
+    ZP_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    ZP_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int st = batch_launch_eager(b, cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    if (st != ZP_OK) return st;
+    ZP_CHECK(e == hipSuccess, ZP_ERR_HIP, "zp_ntt_batch_launch: graph capture failed: %s", hipGetErrorString(e));
+    b->graph = g;
+    ZP_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  }
+  ZP_HIP(hipGraphLaunch(b->exec, s));
+  return ZP_OK;
+}
+
+extern "C" int zp_ntt_batch_destroy(zp_ntt_batch *b) {
+  if (!b) return ZP_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_scratch) (void)hipFree(b->d_scratch);
+  if (b->d_tabs) (void)hipFree(b->d_tabs);
+  delete b;
+  return ZP_OK;
+}
+
+extern "C" int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
+                                   void *stream) {
+  zp_ntt_batch *b = nullptr;
+  ZP_TRY(zp_ntt_batch_create(d_ptrs, log_ns, dirs, count, &b));
+  int st = zp_ntt_batch_launch(b, 0, stream);
+  if (st == ZP_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+    set_last_error("zp_ntt_batch_device: execution failed");
+    st = ZP_ERR_HIP;
+  }
+  zp_ntt_batch_destroy(b);
+  return st;
 }
 
 // ------------------------------------------------------------- utilities --
@@ -397,5 +562,45 @@ extern "C" int zp_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 }
 extern "C" int zp_device_synchronize(void) {
   ZP_HIP(hipDeviceSynchronize());
+  return ZP_OK;
+}
+extern "C" int zp_stream_create(void **stream) {
+  ZP_CHECK(stream != nullptr, ZP_ERR_INVALID_ARG, "zp_stream_create: NULL out");
+  hipStream_t s = nullptr;
+  ZP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return ZP_OK;
+}
+extern "C" int zp_stream_destroy(void *stream) {
+  if (stream) ZP_HIP(hipStreamDestroy((hipStream_t)stream));
+  return ZP_OK;
+}
+extern "C" int zp_stream_synchronize(void *stream) {
+  ZP_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return ZP_OK;
+}
+extern "C" int zp_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream) {
+  ZP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return ZP_OK;
+}
+extern "C" int zp_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream) {
+  ZP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return ZP_OK;
+}
+extern "C" int zp_host_alloc(void **ptr, size_t bytes) {
+  ZP_CHECK(ptr != nullptr, ZP_ERR_INVALID_ARG, "zp_host_alloc: NULL out");
+  if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    set_last_error("zp_host_alloc: %zu bytes failed", bytes);
+    return ZP_ERR_OUT_OF_MEMORY;
+  }
+  return ZP_OK;
+}
+extern "C" int zp_host_free(void *ptr) {
+  if (ptr) ZP_HIP(hipHostFree(ptr));
+  return ZP_OK;
+}
+extern "C" int zp_device_memory_info(size_t *free_bytes, size_t *total_bytes) {
+  ZP_CHECK(free_bytes && total_bytes, ZP_ERR_INVALID_ARG, "zp_device_memory_info: NULL out");
+  ZP_HIP(hipMemGetInfo(free_bytes, total_bytes));
   return ZP_OK;
 }

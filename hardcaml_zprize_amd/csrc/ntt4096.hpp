@@ -1,0 +1,203 @@
+// Register-resident 4096-point sub-DFTs for the 4-step transform (n1 or n2 = 4096).
+//
+// A workgroup holds a tile of T sequences x 4096 points entirely in VGPRs:
+// thread (s, a) keeps the 64 elements j = a + 64 m (m = 0..63) of sequence s.
+//   step A  radix-64 DFT over m in registers; every twiddle is a power of two
+//           (w_64 = OMEGA[6] = 2^39), so it is a shift and a reduction;
+//   step B  twiddle w_4096^(a k') (table in global memory, L1/L2 resident);
+//   step C  transpose through LDS (32-bit halves, 2 rounds) so thread (s, k')
+//           holds the 64 values over a;
+//   step D  radix-64 DFT over a -> output k = k' + 64 k''.
+// Pass 1 (columns of the row-major n1 x n2 matrix): loads/stores 8*T-byte row
+// segments, fuses the 4-step twiddle w_n^(j2 k1) and writes the scratch
+// matrix in the same layout.  Pass 2 (rows): 512-byte coalesced row loads,
+// transposed store X[k2 * n1 + k1] in 8*T-byte segments.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "gl64.hpp"
+
+namespace zp {
+
+// x * 2^S (mod p) for a compile-time S in [0, 192).
+template <int S>
+__device__ __forceinline__ uint64_t gl_shl(uint64_t a) {
+  if constexpr (S == 0) {
+    return a;
+  } else if constexpr (S >= 96) {
+    return gl_neg(gl_shl<S - 96>(a));
+  } else if constexpr (S < 64) {
+    return gl_reduce128(a << S, a >> (64 - S));
+  } else if constexpr (S == 64) {
+    return gl_reduce128(0, a);
+  } else {
+    // a 2^(S-64) fits in 128 bits; times 2^64 is that residue as a high word.
+    return gl_reduce128(0, gl_reduce128(a << (S - 64), a >> (128 - S)));
+  }
+}
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).
+template <typename Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+constexpr int bitrev6(int i) {
+  return ((i & 1) << 5) | ((i & 2) << 3) | ((i & 4) << 1) | ((i & 8) >> 1) | ((i & 16) >> 3) | ((i & 32) >> 5);
+}
+
+// y[k] = sum_m x[m] w_64^(m k) in place, natural order in and out: radix-2
+// DIT on the bit-reversed input, every index a compile-time constant so the
+// 64 values stay in registers.  DIR 0: w_64 = 2^39; DIR 1: w_64^-1 = 2^153.
+template <int DIR>
+__device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
+  uint64_t y[64];
+  static_for<64>([&](auto i) { y[i] = x[bitrev6(i)]; });
+  static_for<6>([&](auto sm1) {
+    constexpr int s = sm1 + 1;
+    constexpr int half = 1 << (s - 1);
+    static_for<32>([&](auto b) {
+      constexpr int j = b % half;
+      constexpr int i0 = (b / half) * 2 * half + j;
+      constexpr int e0 = (39 * j * (64 >> s)) % 192;
+      constexpr int e = DIR ? (192 - e0) % 192 : e0;
+      const uint64_t u = y[i0];
+      const uint64_t t = gl_shl<e>(y[i0 + half]);
+      y[i0] = gl_add(u, t);
+      y[i0 + half] = gl_sub(u, t);
+    });
+  });
+  static_for<64>([&](auto i) { x[i] = y[i]; });
+}
+
+struct Ntt4096Args {
+  const uint64_t *in;
+  uint64_t *out;
+  uint64_t n1, n2;         // matrix rows / columns (n = n1 * n2)
+  const uint64_t *tw4096;  // w_4096^i, i < 4096 (direction-specific)
+  const uint64_t *tw_lo;   // w_n^i, i < 4096 (pass 1 only)
+  const uint64_t *tw_hi;   // w_n^(4096 i)
+  uint64_t scale;          // pass 2: n^-1 for the inverse, else 1
+  uint32_t tiles;          // number of T-wide tiles
+  // Batched form: when set, job blockIdx.y reads in_tab[y] and writes out_tab[y].
+  const uint64_t *const *in_tab;
+  uint64_t *const *out_tab;
+};
+
+// Workgroup id -> tile, so the workgroups dispatched to one XCD (round robin
+// over 8) take adjacent tiles and share their L2 lines.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t g, uint32_t tiles) {
+  const uint32_t per = tiles >> 3;
+  if ((tiles & 7) != 0 || per == 0) return g;
+  return (g & 7) * per + (g >> 3);
+}
+
+// LDS exchange index (u32 words).  The row stride 65 T and the inner
+// strides are chosen so that both the writers (consecutive lanes) and the
+// readers (lanes spread over (s, k')) hit distinct banks.
+//   columns kernel: writers are lanes (s, a) with s fastest -> [k'][a][s];
+//   rows kernel:    writers are lanes (a, s) with a fastest -> [k'][s][a].
+template <int T, bool ROWS>
+__device__ __forceinline__ uint32_t xidx(uint32_t kp, uint32_t a, uint32_t s) {
+  return ROWS ? kp * (65 * T) + s * 65 + a : kp * (65 * T) + a * T + s;
+}
+
+template <int T>
+constexpr size_t ntt4096_lds_bytes() {
+  return (size_t)64 * 65 * T * sizeof(uint32_t);
+}
+
+// x[k'] of thread (s_w, a_w) -> x[a] of thread (s_r, k' = kp_r), in two
+// 32-bit rounds.
+template <int T, bool ROWS>
+__device__ __forceinline__ void exchange(uint64_t (&x)[64], uint32_t *lds, uint32_t s_w, uint32_t a_w, uint32_t s_r,
+                                         uint32_t kp_r) {
+  uint32_t lo[64];
+#pragma unroll
+  for (int k = 0; k < 64; k++) lds[xidx<T, ROWS>(k, a_w, s_w)] = (uint32_t)x[k];
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 64; a++) lo[a] = lds[xidx<T, ROWS>(kp_r, a, s_r)];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 64; k++) lds[xidx<T, ROWS>(k, a_w, s_w)] = (uint32_t)(x[k] >> 32);
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 64; a++) x[a] = ((uint64_t)lds[xidx<T, ROWS>(kp_r, a, s_r)] << 32) | lo[a];
+}
+
+template <int T, int DIR>
+__global__ void __launch_bounds__(64 * T) k_ntt4096_cols(Ntt4096Args args) {
+  extern __shared__ uint32_t lds[];
+  if (args.in_tab) {
+    args.in = args.in_tab[blockIdx.y];
+    args.out = args.out_tab[blockIdx.y];
+  }
+  const uint32_t tile = xcd_tile(blockIdx.x, args.tiles);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t s = tid % T, a = tid / T;  // column within tile, sub-sequence
+  const uint64_t col = (uint64_t)tile * T + s;
+  const uint64_t n2 = args.n2;
+  uint64_t x[64];
+#pragma unroll
+  for (int m = 0; m < 64; m++) x[m] = args.in[(uint64_t)(a + 64 * m) * n2 + col];
+  dft64_regs<DIR>(x);
+#pragma unroll
+  for (int k = 1; k < 64; k++) x[k] = gl_mul(x[k], args.tw4096[a * k]);
+  exchange<T, false>(x, lds, s, a, s, a);
+  dft64_regs<DIR>(x);
+  // x[k''] = output k1 = a + 64 k'' (a now plays k').  4-step twiddle
+  // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.
+  const uint64_t n = args.n1 * n2;
+  const uint64_t e0 = (col * a) & (n - 1);
+  const uint64_t e1 = (col * 64) & (n - 1);
+  uint64_t w = gl_mul(args.tw_hi[e0 >> 12], args.tw_lo[e0 & 4095]);
+  const uint64_t step = gl_mul(args.tw_hi[e1 >> 12], args.tw_lo[e1 & 4095]);
+#pragma unroll
+  for (int k = 0; k < 64; k++) {
+    args.out[(uint64_t)(a + 64 * k) * n2 + col] = gl_mul(x[k], w);
+    w = gl_mul(w, step);
+  }
+}
+
+template <int T, int DIR>
+__global__ void __launch_bounds__(64 * T) k_ntt4096_rows(Ntt4096Args args) {
+  extern __shared__ uint32_t lds[];
+  if (args.in_tab) {
+    args.in = args.in_tab[blockIdx.y];
+    args.out = args.out_tab[blockIdx.y];
+  }
+  const uint32_t tile = xcd_tile(blockIdx.x, args.tiles);
+  const uint32_t tid = threadIdx.x;
+  // load mapping: a wave reads 64 consecutive elements of one row
+  const uint32_t a = tid & 63, r = tid >> 6;
+  const uint64_t row0 = (uint64_t)tile * T;
+  const uint64_t n2 = args.n2;
+  const uint64_t *src = args.in + (row0 + r) * n2;
+  uint64_t x[64];
+#pragma unroll
+  for (int m = 0; m < 64; m++) x[m] = src[a + 64 * m];
+  dft64_regs<DIR>(x);
+#pragma unroll
+  for (int k = 1; k < 64; k++) x[k] = gl_mul(x[k], args.tw4096[a * k]);
+  // after the exchange thread (r2, kp) with r2 fastest, for T-row store runs
+  const uint32_t r2 = tid % T, kp = tid / T;
+  exchange<T, true>(x, lds, r, a, r2, kp);
+  dft64_regs<DIR>(x);
+  const uint64_t n1 = args.n1;
+#pragma unroll
+  for (int k = 0; k < 64; k++) {
+    uint64_t v = x[k];
+    if (args.scale != 1) v = gl_mul(v, args.scale);
+    args.out[(uint64_t)(kp + 64 * k) * n1 + row0 + r2] = v;
+  }
+}
+
+}  // namespace zp

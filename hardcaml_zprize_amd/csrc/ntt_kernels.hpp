@@ -25,6 +25,9 @@ struct DftArgs {
   const uint64_t *tw_lo;   // w_n^i, i < 4096 (4-step twiddle) or NULL
   const uint64_t *tw_hi;   // w_n^(4096 i)
   uint64_t scale;          // 1 = none
+  // Batched form: when set, job blockIdx.y reads in_tab[y] and writes out_tab[y].
+  const uint64_t *const *in_tab;
+  uint64_t *const *out_tab;
 };
 
 static constexpr int DFT_THREADS = 256;
@@ -42,6 +45,10 @@ __device__ __forceinline__ unsigned bitrev_bits(unsigned x, unsigned bits) {
 
 __global__ void __launch_bounds__(DFT_THREADS) k_dft_lds(DftArgs a) {
   extern __shared__ uint64_t tile[];
+  if (a.in_tab) {
+    a.in = a.in_tab[blockIdx.y];
+    a.out = a.out_tab[blockIdx.y];
+  }
   const unsigned L = 1u << a.log_L;
   const unsigned T = 1u << a.log_T;
   const unsigned total = L << a.log_T;
@@ -99,9 +106,9 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft_lds(DftArgs a) {
   }
 }
 
-static inline int launch_dft(const DftArgs &a, uint64_t num_groups, hipStream_t stream) {
+static inline int launch_dft(const DftArgs &a, uint64_t num_groups, hipStream_t stream, unsigned jobs = 1) {
   const size_t lds = (size_t)8 << (a.log_L + a.log_T);
-  hipLaunchKernelGGL(k_dft_lds, dim3((unsigned)num_groups), dim3(DFT_THREADS), lds, stream, a);
+  hipLaunchKernelGGL(k_dft_lds, dim3((unsigned)num_groups, jobs), dim3(DFT_THREADS), lds, stream, a);
   ZP_HIP(hipGetLastError());
   return ZP_OK;
 }

@@ -97,6 +97,45 @@ class MsmContext:
         return out
 
 
+class MsmJob(ctypes.Structure):
+    """zp_msm_job: one MSM over points [0, n) of a resident table."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("n", ctypes.c_size_t), ("d_scalars", ctypes.c_void_p),
+                ("out", ctypes.c_void_p), ("window", ctypes.c_int)]
+
+
+class MsmWorkspace:
+    """Device working memory for one batch at a time (zp_msm_workspace)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("zp_msm_workspace_create", ctypes.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().zp_msm_workspace_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, jobs, options="", stream=None):
+        """jobs: list of (MsmContext, n, d_scalars_ptr[, window]); returns G1_PROJ_DTYPE[len(jobs)]."""
+        out = np.zeros(len(jobs), dtype=G1_PROJ_DTYPE)
+        arr = (MsmJob * len(jobs))()
+        base = out.ctypes.data
+        for i, job in enumerate(jobs):
+            ctx, n, dsc = job[:3]
+            window = int(job[3]) if len(job) > 3 and job[3] else 0
+            arr[i] = MsmJob(ctx.ctx, int(n), int(dsc) if dsc else 0, base + i * G1_PROJ_DTYPE.itemsize, window)
+        opt = options.encode()
+        call("zp_msm_batch_device", self.h, arr, len(jobs), opt, len(opt), _vp(stream))
+        return out
+
+
 class DeviceBuffer:
     """A plain HBM allocation through the library (no torch needed)."""
 
@@ -160,7 +199,7 @@ def debug_digits(scalars, options=""):
     sc = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
     opt = options.encode()
     W = ctypes.c_int()
-    call("zp_debug_msm_digits", opt, len(opt), None, 0, None, ctypes.byref(W))
+    call("zp_debug_msm_digits", opt, len(opt), None, len(sc), None, ctypes.byref(W))
     out = np.zeros((len(sc), W.value), dtype=np.int32)
     call("zp_debug_msm_digits", opt, len(opt), sc.ctypes.data_as(ctypes.c_void_p), len(sc),
          out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(W))

@@ -119,4 +119,32 @@ def ntt_batch_device(ptrs, log_ns, dirs, stream=None):
     call("zp_ntt_batch_device", P, L, D, n, ctypes.c_void_p(stream or 0))
 
 
-__all__ = ["NttGpuDriver", "UserBuffer", "ntt_batch_device", "FORWARD", "INVERSE", "ZpError"]
+class NttBatch:
+    """Many in-place transforms of mixed sizes/directions, grouped into one
+    launch per (size, direction, pass); optionally replayed from a HIP graph."""
+
+    def __init__(self, ptrs, log_ns, dirs):
+        n = len(ptrs)
+        P = (ctypes.c_void_p * n)(*ptrs)
+        L = (ctypes.c_uint * n)(*log_ns)
+        D = (ctypes.c_int * n)(*dirs)
+        h = ctypes.c_void_p()
+        call("zp_ntt_batch_create", P, L, D, n, ctypes.byref(h))
+        self.h = h
+
+    def launch(self, use_graph=True, stream=None):
+        call("zp_ntt_batch_launch", self.h, 1 if use_graph else 0, ctypes.c_void_p(stream or 0))
+
+    def close(self):
+        if self.h:
+            lib().zp_ntt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["NttBatch", "NttGpuDriver", "UserBuffer", "ntt_batch_device", "FORWARD", "INVERSE", "ZpError"]

@@ -1,0 +1,242 @@
+"""Batched and served execution of mixed NTT / MSM jobs on one GPU.
+
+`GpuServer` is a continuous-batching request loop:
+  - the MSM base-point tables stay resident in HBM (MsmContext objects);
+  - every worker thread owns a HIP stream and an MSM workspace, and
+    repeatedly takes all waiting jobs that fit the free HBM budget and runs
+    them as ONE batch (grouped NTT launches + one MSM launch sequence), so
+    requests that arrive while other batches are on the GPU are admitted as
+    soon as a worker and memory free up, and their work overlaps on the device;
+  - the plain form is the same loop with one worker and one job per batch.
+
+No reference counterpart: the reference serialises the device and overlaps
+only PCIe transfers (zprize/ntt/host/driver.cpp:148-231,
+zprize/msm_pippenger/host/driver/driver.cpp:495-600).
+"""
+import ctypes
+import threading
+import time
+from collections import deque
+
+import numpy as np
+
+from ._lib import call, lib
+from .msm import G1_PROJ_DTYPE, MsmWorkspace
+from .ntt import NttBatch
+
+NTT, MSM = "ntt", "msm"
+
+
+class Job:
+    """One request.  kind NTT: data (uint64 n), direction; kind MSM: table, scalars (n, 4)."""
+
+    __slots__ = ("id", "kind", "log_n", "direction", "data", "table", "scalars", "window", "t_arrival",
+                 "t_start", "t_done", "result", "error", "_event")
+
+    def __init__(self, kind, **kw):
+        self.kind = kind
+        self.id = kw.get("id", 0)
+        self.log_n = kw.get("log_n", 0)
+        self.direction = kw.get("direction", 0)
+        self.data = kw.get("data")
+        self.table = kw.get("table", 0)
+        self.scalars = kw.get("scalars")
+        self.window = kw.get("window", 0)
+        self.t_arrival = kw.get("t_arrival", 0.0)
+        self.t_start = self.t_done = None
+        self.result = None
+        self.error = None
+        self._event = threading.Event()
+
+    def hbm_bytes(self):
+        n = 1 << self.log_n
+        if self.kind == NTT:
+            return 16 * n  # vector + 4-step scratch
+        # scalars + sorted entries (<= 32 windows) + work units + buckets, generous
+        return 32 * n + 4 * 32 * n + 64 * n + (1 << 26)
+
+    def wait(self, timeout=None):
+        self._event.wait(timeout)
+        return self
+
+
+class _DevicePool:
+    """Reusable device allocations keyed by size (one pool per worker)."""
+
+    def __init__(self):
+        self.free = {}
+
+    def get(self, nbytes):
+        lst = self.free.get(nbytes)
+        if lst:
+            return lst.pop()
+        p = ctypes.c_void_p()
+        call("zp_device_malloc", ctypes.byref(p), max(nbytes, 1))
+        return p.value
+
+    def put(self, nbytes, ptr):
+        self.free.setdefault(nbytes, []).append(ptr)
+
+    def release(self):
+        for lst in self.free.values():
+            for p in lst:
+                lib().zp_device_free(ctypes.c_void_p(p))
+        self.free.clear()
+
+
+class GpuServer:
+    def __init__(self, tables, hbm_budget=200e9, workers=2, max_batch_jobs=256, batching=True):
+        self.tables = tables
+        self.budget = float(hbm_budget)
+        self.in_use = 0.0
+        self.batching = batching
+        self.max_batch = max_batch_jobs if batching else 1
+        self.queue = deque()
+        self.cv = threading.Condition()
+        self.stop = False
+        self.batches = 0
+        self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
+        for t in self.threads:
+            t.start()
+
+    # ---------------------------------------------------------------- API
+    def submit(self, job):
+        if job.hbm_bytes() > self.budget:
+            raise ValueError(f"job {job.id} needs {job.hbm_bytes()} bytes > HBM budget {self.budget}")
+        with self.cv:
+            self.queue.append(job)
+            self.cv.notify()
+        return job
+
+    def close(self):
+        with self.cv:
+            self.stop = True
+            self.cv.notify_all()
+        for t in self.threads:
+            t.join()
+
+    # ------------------------------------------------------------- worker
+    def _take(self):
+        """Admit waiting jobs in arrival order while they fit the free budget."""
+        with self.cv:
+            while not self.queue and not self.stop:
+                self.cv.wait()
+            if not self.queue:
+                return None
+            batch, need = [], 0.0
+            while self.queue and len(batch) < self.max_batch:
+                b = self.queue[0].hbm_bytes()
+                if batch and self.in_use + need + b > self.budget:
+                    break
+                if not batch and self.in_use + b > self.budget:
+                    # wait for memory to free up
+                    self.cv.wait(0.01)
+                    continue
+                batch.append(self.queue.popleft())
+                need += b
+            self.in_use += need
+            return batch, need
+
+    def _release(self, need):
+        with self.cv:
+            self.in_use -= need
+            self.cv.notify_all()
+
+    def _worker(self):
+        s = ctypes.c_void_p()
+        call("zp_stream_create", ctypes.byref(s))
+        stream = s.value
+        ws = MsmWorkspace()
+        pool = _DevicePool()
+        try:
+            while True:
+                taken = self._take()
+                if taken is None:
+                    return
+                batch, need = taken
+                t0 = time.perf_counter()
+                for j in batch:
+                    j.t_start = t0
+                try:
+                    self._run_batch(batch, stream, ws, pool)
+                except Exception as e:  # surfaced on the job
+                    for j in batch:
+                        j.error = e
+                t1 = time.perf_counter()
+                self._release(need)
+                with self.cv:
+                    self.batches += 1
+                for j in batch:
+                    j.t_done = t1
+                    j._event.set()
+        finally:
+            ws.close()
+            pool.release()
+            lib().zp_stream_destroy(ctypes.c_void_p(stream))
+
+    def _run_batch(self, batch, stream, ws, pool):
+        ntt_jobs = [j for j in batch if j.kind == NTT]
+        msm_jobs = [j for j in batch if j.kind == MSM]
+        held = []
+        sp = ctypes.c_void_p(stream)
+        # inputs to HBM
+        for j in ntt_jobs:
+            nb = 8 << j.log_n
+            p = pool.get(nb)
+            held.append((nb, p))
+            j.result = np.ascontiguousarray(j.data, dtype=np.uint64)
+            call("zp_memcpy_h2d_async", ctypes.c_void_p(p), j.result.ctypes.data_as(ctypes.c_void_p), nb, sp)
+        msm_ptrs = []
+        for j in msm_jobs:
+            nb = 32 * max(len(j.scalars), 1)
+            p = pool.get(nb)
+            held.append((nb, p))
+            sc = np.ascontiguousarray(j.scalars, dtype=np.uint64)
+            call("zp_memcpy_h2d_async", ctypes.c_void_p(p), sc.ctypes.data_as(ctypes.c_void_p), sc.nbytes, sp)
+            msm_ptrs.append(p)
+        ntt_ptrs = [p for (nb, p) in held[:len(ntt_jobs)]]
+        b = None
+        if ntt_jobs:
+            b = NttBatch(ntt_ptrs, [j.log_n for j in ntt_jobs], [j.direction for j in ntt_jobs])
+            b.launch(use_graph=False, stream=stream)
+        if msm_jobs:
+            specs = [(self.tables[j.table], len(j.scalars), p, j.window) for j, p in zip(msm_jobs, msm_ptrs)]
+            out = ws.run(specs, stream=stream)
+            for j, o in zip(msm_jobs, out):
+                j.result = o
+        for j, p in zip(ntt_jobs, ntt_ptrs):
+            res = np.empty(1 << j.log_n, dtype=np.uint64)
+            call("zp_memcpy_d2h_async", res.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(p), res.nbytes, sp)
+            j.result = res
+        call("zp_stream_synchronize", sp)
+        if b is not None:
+            b.close()
+        for nb, p in held:
+            pool.put(nb, p)
+
+
+def run_stream(server, jobs, arrivals):
+    """Submit jobs at the given arrival offsets (seconds from now), wait for all.
+    Returns (makespan_s, latencies_s)."""
+    t0 = time.perf_counter()
+    for j, a in zip(jobs, arrivals):
+        now = time.perf_counter() - t0
+        if a > now:
+            time.sleep(a - now)
+        j.t_arrival = time.perf_counter()
+        server.submit(j)
+    for j in jobs:
+        j.wait()
+    t1 = max(j.t_done for j in jobs)
+    for j in jobs:
+        if j.error is not None:
+            raise j.error
+    return t1 - t0, [j.t_done - j.t_arrival for j in jobs]
+
+
+def percentiles(values, ps=(50, 99)):
+    v = np.sort(np.asarray(values))
+    return {f"p{p}": float(np.percentile(v, p)) for p in ps}
+
+
+__all__ = ["GpuServer", "Job", "NTT", "MSM", "run_stream", "percentiles", "G1_PROJ_DTYPE"]

@@ -87,11 +87,18 @@ int zp_ntt_wait(zp_ntt_plan *plan, int buffer_id);
 int zp_ntt_free_buffer(zp_ntt_plan *plan, int buffer_id); /* ZP_ERR_STATE if not in use */
 
 /* Batched form: `count` transforms of mixed sizes and directions, all
- * resident in HBM, in place, issued as one launch sequence on `stream`
- * (captured once into a HIP graph per distinct job list when use_graph != 0).
- * log_ns[i] in [1, 24]; d_ptrs[i] device pointers; dirs[i] in {0,1}.
- * No reference counterpart: the reference runs one transform per enqueue
- * (bench_throughput.cpp:61-107). */
+ * resident in HBM, in place.  Jobs are grouped by (log_n, dir) and each
+ * group runs as one launch per pass with the job index on grid.y; with
+ * use_graph != 0 the whole sequence is captured into a HIP graph at the
+ * first launch and replayed afterwards.  log_ns[i] in [1, 24]; d_ptrs[i]
+ * device pointers; dirs[i] in {0,1}.  No reference counterpart: the
+ * reference runs one transform per enqueue (bench_throughput.cpp:61-107). */
+typedef struct zp_ntt_batch zp_ntt_batch;
+int zp_ntt_batch_create(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
+                        zp_ntt_batch **out);
+int zp_ntt_batch_launch(zp_ntt_batch *batch, int use_graph, void *stream);
+int zp_ntt_batch_destroy(zp_ntt_batch *batch);
+/* create + eager launch + synchronize + destroy. */
 int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
                         void *stream);
 
@@ -140,6 +147,27 @@ int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_b
 int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_projective_t *out, const biginteger256_t *d_scalars,
                               void *stream);
 size_t zp_msm_num_points(const zp_msm_context *ctx);
+
+/* Batched form: many MSMs of mixed sizes, each over a prefix of a resident
+ * table (tables may differ between jobs), run as ONE launch sequence: the
+ * jobs' windows and buckets are concatenated, so digits, counting sort,
+ * bucket sums and window reductions of every job share each kernel launch.
+ * Blocks until every out is written.  One workspace runs one batch at a
+ * time; concurrent batches use distinct workspaces and streams.  No
+ * reference counterpart (the reference runs one MSM batch after another,
+ * driver.cpp:495-600). */
+typedef struct {
+  zp_msm_context *ctx;               /* resident point table */
+  size_t n;                          /* use points [0, n) */
+  const biginteger256_t *d_scalars;  /* n scalars in HBM */
+  g1_projective_t *out;              /* host output, one point */
+  int window;                        /* window bits in [8, 24]; 0 = size-class default */
+} zp_msm_job;
+typedef struct zp_msm_workspace zp_msm_workspace;
+int zp_msm_workspace_create(zp_msm_workspace **out);
+int zp_msm_workspace_destroy(zp_msm_workspace *ws);
+int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs, size_t count, const char *options,
+                        size_t options_len, void *stream);
 /* Number of loaded points without a twisted-Edwards image (handled by the
  * Weierstrass fallback, driver.cpp:349-372). */
 size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx);
@@ -174,6 +202,16 @@ int zp_device_free(void *ptr);
 int zp_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int zp_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int zp_device_synchronize(void);
+/* Streams (default priority), pinned host memory and async copies for hosts
+ * that drive the library without another GPU runtime. */
+int zp_stream_create(void **stream);
+int zp_stream_destroy(void *stream);
+int zp_stream_synchronize(void *stream);
+int zp_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream);
+int zp_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream);
+int zp_host_alloc(void **ptr, size_t bytes);
+int zp_host_free(void *ptr);
+int zp_device_memory_info(size_t *free_bytes, size_t *total_bytes);
 
 #ifdef __cplusplus
 }

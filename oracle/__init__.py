@@ -65,6 +65,7 @@ def lib():
         L.orc_g1_generator.argtypes = [vp]
         L.orc_g1_mul_generator_many.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
         L.orc_g1_projective_to_affine.argtypes = [vp, vp]
+        L.orc_dot_mod_r.argtypes = [vp, vp, ctypes.c_size_t, vp]
         _lib = L
     return _lib
 
@@ -200,6 +201,16 @@ def mul_generator_many(ks, threads=8):
     out = np.zeros(len(ks), dtype=G1_AFFINE_DTYPE)
     lib().orc_g1_mul_generator_many(_ptr(out), _ptr(ks), len(ks), threads)
     return out
+
+
+def dot_mod_r(s, k):
+    """sum_i s_i k_i mod r for uint64 (n, 4) arrays."""
+    s = np.ascontiguousarray(s, dtype=np.uint64)
+    k = np.ascontiguousarray(k, dtype=np.uint64)
+    assert s.shape == k.shape and s.shape[1] == 4
+    out = np.zeros(4, dtype=np.uint64)
+    lib().orc_dot_mod_r(_ptr(s), _ptr(k), len(s), _ptr(out))
+    return words_to_int(out)
 
 
 def g1_add(a, b):

@@ -893,3 +893,58 @@ void orc_msm(const orc_g1_affine *points, size_t npoints, const orc_bigint256 *s
   free(conv);
   free(dig);
 }
+
+/* sum_i s_i * k_i mod r for 256-bit little-endian integers (closed-form MSM
+ * checks: with P_i = k_i G the MSM equals (sum s_i k_i mod r) G). */
+void orc_dot_mod_r(const orc_bigint256 *s, const orc_bigint256 *k, size_t n, uint64_t out[4]) {
+  static const uint64_t R_W[4] = {0x0a11800000000001ull, 0x59aa76fed0000001ull, 0x60b44d1e5c37b001ull,
+                                  0x12ab655e9a2ca556ull};
+  uint64_t acc[10] = {0};
+  for (size_t i = 0; i < n; i++) {
+    uint64_t prod[8] = {0};
+    for (int a = 0; a < 4; a++) {
+      u128 c = 0;
+      for (int b = 0; b < 4; b++) {
+        c = (u128)s[i].data[a] * k[i].data[b] + prod[a + b] + (uint64_t)(c >> 64);
+        prod[a + b] = (uint64_t)c;
+      }
+      prod[a + 4] = (uint64_t)(c >> 64);
+    }
+    u128 c = 0;
+    for (int j = 0; j < 10; j++) {
+      c = (u128)acc[j] + (j < 8 ? prod[j] : 0) + (uint64_t)(c >> 64);
+      acc[j] = (uint64_t)c;
+    }
+  }
+  /* acc mod r by shift-and-subtract from the top bit down. */
+  uint64_t rem[5] = {0};
+  for (int bit = 639; bit >= 0; bit--) {
+    /* rem = rem * 2 + bit */
+    uint64_t carry = (acc[bit / 64] >> (bit % 64)) & 1;
+    for (int j = 0; j < 5; j++) {
+      uint64_t nc = rem[j] >> 63;
+      rem[j] = (rem[j] << 1) | carry;
+      carry = nc;
+    }
+    /* if rem >= r: rem -= r */
+    int ge = rem[4] != 0;
+    if (!ge) {
+      ge = 1;
+      for (int j = 3; j >= 0; j--) {
+        if (rem[j] != R_W[j]) {
+          ge = rem[j] > R_W[j];
+          break;
+        }
+      }
+    }
+    if (ge) {
+      u128 br = 0;
+      for (int j = 0; j < 5; j++) {
+        u128 d = (u128)rem[j] - (j < 4 ? R_W[j] : 0) - br;
+        rem[j] = (uint64_t)d;
+        br = (d >> 64) & 1;
+      }
+    }
+  }
+  for (int j = 0; j < 4; j++) out[j] = rem[j];
+}

@@ -85,6 +85,8 @@ void orc_g1_scalar_mul(orc_g1_affine *out, const orc_g1_affine *p, const orc_big
 void orc_g1_generator(orc_g1_affine *out);
 /* P_i = k_i * G for a batch (fixed-base generation used by tests). */
 void orc_g1_mul_generator_many(orc_g1_affine *out, const orc_bigint256 *k, size_t n, int threads);
+/* sum_i s_i k_i mod r (closed-form MSM check). */
+void orc_dot_mod_r(const orc_bigint256 *s, const orc_bigint256 *k, size_t n, uint64_t out[4]);
 /* Projective (x*R, y*R, R)/(z=0) -> affine with infinity flag. */
 void orc_g1_projective_to_affine(orc_g1_affine *out, const orc_g1_projective *p);
 

@@ -224,3 +224,40 @@ def test_large_closed_form():
     assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(oracle.generator(), e))
     dk.free()
     dp.free()
+
+
+def test_batch_mixed_jobs_two_tables():
+    """Batched form: jobs of mixed sizes over two resident tables in one
+    launch sequence; every result equals the job run alone."""
+    _say("msm batch mixed")
+    rng = random.Random(77)
+    g = oracle.generator()
+    tables = []
+    for t in range(2):
+        n = 3000 + 1000 * t
+        ks = [rng.randrange(1, R) for _ in range(n)]
+        tables.append((ks, zmsm.MsmContext(zmsm.generate_points(_words(ks, 4)))))
+    jobs, expect, bufs = [], [], []
+    for j in range(9):
+        ks, ctx = tables[j % 2]
+        n = rng.choice([0, 1, 17, 256, 1000, len(ks)])
+        s = [rng.randrange(R) for _ in range(max(n, 1))]
+        b = zmsm.DeviceBuffer(32 * max(n, 1)).upload(_words(s, 4))
+        bufs.append(b)
+        jobs.append((ctx, n, b.ptr))
+        e = sum(a * k for a, k in zip(s[:n], ks[:n])) % R
+        expect.append(oracle.g1_scalar_mul(g, e))
+    ws = zmsm.MsmWorkspace()
+    try:
+        for rep in range(2):
+            out = ws.run(jobs)
+            for j, (o, e) in enumerate(zip(out, expect)):
+                assert oracle.affine_equal(oracle.projective_to_affine(o), e), (rep, j)
+        alone = [ctx.mult_prefix_device(n, p) for ctx, n, p in jobs]
+        assert all(a.tobytes() == b.tobytes() for a, b in zip(alone, out))
+    finally:
+        ws.close()
+        for b in bufs:
+            b.free()
+        for _, ctx in tables:
+            ctx.close()

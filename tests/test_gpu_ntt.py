@@ -113,3 +113,45 @@ def test_batch_mixed_sizes():
     finally:
         for p in ptrs:
             call("zp_device_free", ctypes.c_void_p(p))
+
+
+def test_batch_graph_replays():
+    """Graph-captured batch: every replay checked against the oracle, with
+    eager launches and host synchronisations in between."""
+    _say("ntt batch graph")
+    rng = np.random.default_rng(8)
+    log_ns = [10, 12, 13, 16, 16, 20, 24, 11, 18]
+    dirs = [int(x) for x in rng.integers(0, 2, size=len(log_ns))]
+    inputs = [_rand(1 << k, 300 + i) for i, k in enumerate(log_ns)]
+    expect = [oracle.ntt(a, inverse=bool(d)) for a, d in zip(inputs, dirs)]
+    ptrs = []
+    for a in inputs:
+        p = ctypes.c_void_p()
+        call("zp_device_malloc", ctypes.byref(p), a.nbytes)
+        ptrs.append(p.value)
+
+    def upload():
+        for p, a in zip(ptrs, inputs):
+            call("zp_memcpy_h2d", ctypes.c_void_p(p), a.ctypes.data, a.nbytes)
+
+    def check(tag):
+        for p, a, e in zip(ptrs, inputs, expect):
+            out = np.empty_like(a)
+            call("zp_memcpy_d2h", out.ctypes.data, ctypes.c_void_p(p), a.nbytes)
+            assert np.array_equal(out, e), tag
+
+    b = zntt.NttBatch(ptrs, log_ns, dirs)
+    try:
+        for rep in range(3):
+            upload()
+            b.launch(use_graph=True)
+            call("zp_device_synchronize")
+            check(f"graph replay {rep}")
+            upload()
+            b.launch(use_graph=False)
+            call("zp_device_synchronize")
+            check(f"eager {rep}")
+    finally:
+        b.close()
+        for p in ptrs:
+            call("zp_device_free", ctypes.c_void_p(p))
