@@ -31,6 +31,10 @@ def parse():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--msm-log-n", type=int, default=26)
     ap.add_argument("--msm-steps", type=int, default=3)
+    ap.add_argument("--msm-warmup", type=int, default=1)
+    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-load", type=float, default=2.0)
+    ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--skip-msm", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
@@ -89,6 +93,253 @@ def bench_ntt(args, torch):
     return res
 
 
+R_WORDS = (0x0A11800000000001, 0x59AA76FED0000001, 0x60B44D1E5C37B001, 0x12AB655E9A2CA556)
+
+
+def rand_scalars(rng, n, nonzero=False):
+    """n uniform 253-bit integers below r (rejection sampling), uint64 (n, 4)."""
+    import numpy as np
+
+    out = np.empty((n, 4), dtype=np.uint64)
+    filled = 0
+    r = [np.uint64(w) for w in R_WORDS]
+    while filled < n:
+        m = int((n - filled) * 1.8) + 64
+        c = rng.integers(0, 2**64, size=(m, 4), dtype=np.uint64)
+        c[:, 3] &= np.uint64((1 << 61) - 1)
+        lt = (c[:, 3] < r[3]) | ((c[:, 3] == r[3]) & ((c[:, 2] < r[2]) | ((c[:, 2] == r[2]) & (
+            (c[:, 1] < r[1]) | ((c[:, 1] == r[1]) & (c[:, 0] < r[0]))))))
+        if nonzero:
+            lt &= c.any(axis=1)
+        c = c[lt]
+        take = min(len(c), n - filled)
+        out[filled:filled + take] = c[:take]
+        filled += take
+    return out
+
+
+def make_table(zmsm, rng, log_n):
+    """P_i = k_i G generated on the GPU, loaded as a resident MSM table."""
+    import numpy as np
+
+    n = 1 << log_n
+    ks = rand_scalars(rng, n, nonzero=True)
+    dk = zmsm.DeviceBuffer(32 * n).upload(ks)
+    dp = zmsm.DeviceBuffer(104 * n)
+    t0 = time.perf_counter()
+    zmsm.generate_points_device(dk.ptr, dp.ptr, n)
+    from hardcaml_zprize_amd._lib import call
+    call("zp_device_synchronize")
+    t1 = time.perf_counter()
+    ctx = zmsm.MsmContext((dp.ptr, n))
+    t2 = time.perf_counter()
+    dk.free()
+    dp.free()
+    del ks
+    return ctx, t1 - t0, t2 - t1
+
+
+def bench_msm(args, probe):
+    import numpy as np
+
+    from hardcaml_zprize_amd import msm as zmsm
+
+    log_n = args.msm_log_n
+    n = 1 << log_n
+    rng = np.random.default_rng(args.seed + 1000)
+    log(f"MSM 2^{log_n}: generating base points on the GPU")
+    ctx, gen_s, init_s = make_table(zmsm, rng, log_n)
+    log(f"MSM points generated in {gen_s:.2f} s, msm_init (resident precompute) {init_s:.2f} s")
+    dsc = zmsm.DeviceBuffer(32 * n)
+    dev_ms, host_ms = [], []
+    for call_i in range(args.msm_warmup + args.msm_steps):
+        sc = rand_scalars(np.random.default_rng([args.seed, call_i]), n)  # fresh per call
+        t0 = time.perf_counter()
+        ctx.mult(sc, 1)  # msm_mult with host scalars (H2D included)
+        t1 = time.perf_counter()
+        dsc.upload(sc)
+        t2 = time.perf_counter()
+        ctx.mult_device(dsc.ptr, 1)  # scalars resident in HBM
+        t3 = time.perf_counter()
+        if call_i >= args.msm_warmup:
+            host_ms.append((t1 - t0) * 1e3)
+            dev_ms.append((t3 - t2) * 1e3)
+        log(f"MSM call {call_i}: msm_mult {1e3 * (t1 - t0):.1f} ms, resident scalars {1e3 * (t3 - t2):.1f} ms")
+    dsc.free()
+    ctx.close()
+    ms = float(np.median(dev_ms))
+    work = 140 * n * 288  # reference algorithm's v_mad_u64_u32 lane-ops (20 windows x 7 Fq muls x 288 MACs)
+    peak = probe["mad_u64_u32_lane_ops_per_s"]
+    return {"n": n, "ms": ms, "ms_msm_mult_host_scalars": float(np.median(host_ms)), "point_gen_s": gen_s,
+            "init_s": init_s,
+            "roofline": {"bound": "valu", "achieved": work / (ms * 1e-3), "peak": peak, "unit": "mad_u64_u32 lane-op/s",
+                         "frac": work / (ms * 1e-3) / peak,
+                         "basis": "140 N Fq muls x 288 v_mad_u64_u32 (reference algorithm) / resident-scalar MSM time"}}
+
+
+def bench_batched(args):
+    """256 NTTs (log2 n ~ U[10,20]) + 64 MSMs (log2 N ~ U[12,22]) over one 2^22 table:
+    plain one-at-a-time calls vs one batch (graph-replayed NTT groups + one MSM launch sequence)."""
+    import threading
+
+    import numpy as np
+
+    from hardcaml_zprize_amd import msm as zmsm
+    from hardcaml_zprize_amd import ntt as zntt
+    from hardcaml_zprize_amd._lib import call
+
+    rng = np.random.default_rng(args.seed + 2000)
+    ctx, _, _ = make_table(zmsm, rng, 22)
+    P = 2**64 - 2**32 + 1
+    ntt_logs = [int(x) for x in rng.integers(10, 21, size=256)]
+    ntt_dirs = [int(x) for x in rng.integers(0, 2, size=256)]
+    msm_logs = [int(x) for x in rng.integers(12, 23, size=64)]
+    ntt_bufs = []
+    for k in ntt_logs:
+        b = zmsm.DeviceBuffer(8 << k).upload(rng.integers(0, P, size=1 << k, dtype=np.uint64))
+        ntt_bufs.append(b)
+    msm_bufs = [zmsm.DeviceBuffer(32 << k).upload(rand_scalars(rng, 1 << k)) for k in msm_logs]
+    plans = {}
+    for k in set(ntt_logs):
+        plans[k] = zntt.NttGpuDriver(k, num_buffers=1)
+
+    def plain():
+        lat = []
+        t0 = time.perf_counter()
+        order = [("n", i) for i in range(256)] + [("m", i) for i in range(64)]
+        for kind, i in order:
+            if kind == "n":
+                plans[ntt_logs[i]].inplace_device(ntt_bufs[i].ptr, ntt_dirs[i])
+                call("zp_device_synchronize")
+            else:
+                ctx.mult_prefix_device(1 << msm_logs[i], msm_bufs[i].ptr)
+            lat.append(time.perf_counter() - t0)
+        return time.perf_counter() - t0, lat
+
+    nb = zntt.NttBatch([b.ptr for b in ntt_bufs], ntt_logs, ntt_dirs)
+    ws = zmsm.MsmWorkspace()
+    s1, s2 = ctypes.c_void_p(), ctypes.c_void_p()
+    call("zp_stream_create", ctypes.byref(s1))
+    call("zp_stream_create", ctypes.byref(s2))
+    msm_specs = [(ctx, 1 << k, b.ptr) for k, b in zip(msm_logs, msm_bufs)]
+
+    def batched():
+        t0 = time.perf_counter()
+        res = {}
+
+        def run_msm():
+            res["msm"] = ws.run(msm_specs, stream=s2.value)
+            res["t_msm"] = time.perf_counter() - t0
+
+        th = threading.Thread(target=run_msm)
+        th.start()
+        nb.launch(use_graph=True, stream=s1.value)
+        call("zp_stream_synchronize", s1)
+        t_ntt = time.perf_counter() - t0
+        th.join()
+        total = time.perf_counter() - t0
+        return total, [t_ntt] * 256 + [res["t_msm"]] * 64, res["msm"]
+
+    plain()  # warm-up (tables, plans, workspaces)
+    batched()
+    reps = 3
+    tp = [plain() for _ in range(reps)]
+    tb = [batched() for _ in range(reps)]
+    # invariance: each MSM result in the batch equals the job run alone
+    alone = [ctx.mult_prefix_device(n, p) for _, n, p in msm_specs]
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(alone, tb[-1][2]))
+    plain_t = float(np.median([t for t, _ in tp]))
+    batch_t = float(np.median([t for t, _, _ in tb]))
+    lat_p = np.array(tp[len(tp) // 2][1])
+    lat_b = np.array(tb[len(tb) // 2][1])
+    nb.close()
+    ws.close()
+    for p in plans.values():
+        p.close()
+    for b in ntt_bufs + msm_bufs:
+        b.free()
+    call("zp_stream_destroy", s1)
+    call("zp_stream_destroy", s2)
+    ctx.close()
+    jobs = 320
+    return {"jobs": jobs, "plain_jobs_per_s": jobs / plain_t, "batched_jobs_per_s": jobs / batch_t,
+            "speedup": plain_t / batch_t, "plain_s": plain_t, "batched_s": batch_t,
+            "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3), "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
+            "batched_p50_ms": float(np.percentile(lat_b, 50) * 1e3),
+            "batched_p99_ms": float(np.percentile(lat_b, 99) * 1e3),
+            "msm_results_equal_alone": bool(same),
+            "note": "NTT inputs resident, transformed in place each repetition; MSM scalars resident; "
+                    "latency measured from batch start"}
+
+
+def bench_served(args):
+    """Poisson stream: 70% MSM (log2 N in [14,24], 3 resident 2^24 tables), 30% NTT
+    (log2 n in [12,24], fwd/inv), HBM budget 200 GB; plain one-at-a-time vs continuous batching."""
+    import numpy as np
+
+    from hardcaml_zprize_amd import msm as zmsm
+    from hardcaml_zprize_amd import serve
+
+    rng = np.random.default_rng(args.seed + 3000)
+    tables = []
+    for t in range(3):
+        ctx, _, _ = make_table(zmsm, rng, 24)
+        tables.append(ctx)
+    P = 2**64 - 2**32 + 1
+    pool = rand_scalars(rng, 1 << 24)
+    ntt_pool = rng.integers(0, P, size=1 << 24, dtype=np.uint64)
+    njobs = args.serve_jobs
+    specs = []
+    for i in range(njobs):
+        if rng.random() < 0.7:
+            k = int(rng.integers(14, 25))
+            off = int(rng.integers(0, (1 << 24) - (1 << k) + 1))
+            specs.append(dict(kind=serve.MSM, log_n=k, table=int(rng.integers(0, 3)), off=off))
+        else:
+            k = int(rng.integers(12, 25))
+            specs.append(dict(kind=serve.NTT, log_n=k, direction=int(rng.integers(0, 2))))
+
+    def jobs():
+        out = []
+        for i, sp in enumerate(specs):
+            if sp["kind"] == serve.MSM:
+                n = 1 << sp["log_n"]
+                out.append(serve.Job(serve.MSM, id=i, log_n=sp["log_n"], table=sp["table"],
+                                     scalars=pool[sp["off"]:sp["off"] + n]))
+            else:
+                out.append(serve.Job(serve.NTT, id=i, log_n=sp["log_n"], direction=sp["direction"],
+                                     data=ntt_pool[:1 << sp["log_n"]]))
+        return out
+
+    # calibrate: plain server with every job queued at t = 0
+    plain = serve.GpuServer(tables, hbm_budget=200e9, workers=1, batching=False)
+    serve.run_stream(plain, jobs()[:8], [0.0] * 8)  # warm-up
+    cal_s, _ = serve.run_stream(plain, jobs(), [0.0] * njobs)
+    rate = args.serve_load * njobs / cal_s  # offered load relative to plain capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+    arrivals -= arrivals[0]
+    jp = jobs()
+    plain_s, plain_lat = serve.run_stream(plain, jp, list(arrivals))
+    plain.close()
+    served = serve.GpuServer(tables, hbm_budget=200e9, workers=args.serve_workers, batching=True)
+    serve.run_stream(served, jobs()[:8], [0.0] * 8)  # warm-up
+    js = jobs()
+    served_s, served_lat = serve.run_stream(served, js, list(arrivals))
+    batches = served.batches
+    served.close()
+    same = all((a.result.tobytes() == b.result.tobytes()) for a, b in zip(jp, js))
+    for t in tables:
+        t.close()
+    return {"jobs": njobs, "offered_rate_jobs_per_s": rate, "plain_capacity_jobs_per_s": njobs / cal_s,
+            "plain_jobs_per_s": njobs / plain_s, "served_jobs_per_s": njobs / served_s,
+            "speedup": plain_s / served_s,
+            "plain_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(plain_lat).items()},
+            "served_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(served_lat).items()},
+            "served_batches": batches, "workers": args.serve_workers, "hbm_budget_gb": 200,
+            "results_equal_plain": bool(same),
+            "note": "host inputs (H2D/D2H inside each job); MSM scalars are slices of one seeded pool"}
+
+
 def probe_rates():
     from hardcaml_zprize_amd._lib import call
 
@@ -121,6 +372,27 @@ def cpu_baseline_ntt(log_n, seed, budget_s=12.0):
     return {"value": 16 * n / per / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
             "sample": f"{runs} x 2^{log_n} forward NTT, restated zprize/ntt/host/reference.cpp ntt_reference, "
                       f"1 thread, {per:.3f} s each"}
+
+
+def cpu_baseline_msm(seed, log_n=15):
+    """Restated reference host algorithm (oracle: 20 signed windows, twisted-Edwards
+    buckets, running-sum reduction), windows in parallel on the host cores."""
+    import numpy as np
+
+    import oracle  # checker / CPU baseline only
+
+    threads = min(8, os.cpu_count() or 1)
+    rng = np.random.default_rng(seed + 77)
+    n = 1 << log_n
+    ks = rand_scalars(rng, n, nonzero=True)
+    pts = oracle.mul_generator_many(ks, threads=threads)
+    sc = rand_scalars(rng, n)
+    t0 = time.perf_counter()
+    oracle.msm(pts, sc, 1, threads=threads)
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"one 2^{log_n}-point MSM in {el:.2f} s; extrapolated 2^26 MSM {el * (1 << 26) / n:.1f} s",
+            "extrapolated_2p26_s": el * (1 << 26) / n}
 
 
 def main():
@@ -166,10 +438,21 @@ def main():
                      "basis": "16*n algorithmic bytes per transform / mean transform time (2 launches of k_dft_lds)"},
     }
     out["ntt"] = {"median_ms": ntt["median_ms"], "min_ms": ntt["min_ms"]}
-    out["valu_probe"] = probe_rates()
+    out["valu_probe"] = probe = probe_rates()
+    if not args.skip_msm:
+        out["msm"] = bench_msm(args, probe)
+        log(f"MSM 2^{args.msm_log_n}: {out['msm']['ms']:.1f} ms (resident scalars)")
+    if not args.skip_batch:
+        out["batched"] = bench_batched(args)
+        log(f"batched: {out['batched']}")
+    if not args.skip_serve:
+        out["served"] = bench_served(args)
+        log(f"served: {out['served']}")
     if not args.skip_cpu:
         log("CPU baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_baseline_ntt(args.log_n, args.seed)
+        if not args.skip_msm:
+            out["msm"]["cpu_baseline"] = cpu_baseline_msm(args.seed)
     print(json.dumps(out), flush=True)
 
 

@@ -37,18 +37,23 @@ ZP_HD Xyzt ec_identity(const FConsts &K) {
 }
 
 // acc += (neg ? -P : P), P in precomputed form.  7 multiplications.
+// Sums feeding a multiply stay un-normalized where the other factor is
+// normalized (f_mul bound), so only E, H and G take a carry pass.
 ZP_HD void ec_madd(Xyzt &a, const PrePt &p, bool neg, const FConsts &K) {
   const F &p0 = neg ? p.ypx : p.ymx;
   const F &p1 = neg ? p.ymx : p.ypx;
-  const F A = f_mul(f_sub_kq(a.y, a.x, K.q2), p0);  // (Y - X)(y2 - x2) ; Y-X < 4q
-  const F B = f_mul(f_add(a.y, a.x), p1);           // (Y + X)(y2 + x2) ; < 4q
+  const F A = f_mul(f_sub_nc(a.y, a.x, K.q4), p0);  // (Y - X)(y2 - x2) ; Y + 4q - X < 6q
+  const F B = f_mul(f_add_nc(a.y, a.x), p1);        // (Y + X)(y2 + x2) ; < 4q
   const F C = f_mul(a.t, p.t2d);                    // T 2d t2
-  const F D = f_add(a.z, a.z);                      // 2 Z ; < 4q
-  const F E = f_sub_kq(B, A, K.q2);                 // < 4q
-  const F H = f_add(B, A);                          // < 4q
-  // -P negates t2: F and G swap.
-  const F Fm = neg ? f_add(D, C) : f_sub_kq(D, C, K.q2);  // < 6q
-  const F Gm = neg ? f_sub_kq(D, C, K.q2) : f_add(D, C);  // < 6q
+  const F D = f_add_nc(a.z, a.z);                   // 2 Z ; < 4q, limbs < 2^29
+  F E = f_sub_nc(B, A, K.q4);                       // < 6q
+  F H = f_add_nc(B, A);                             // < 4q
+  f_carry(E);
+  f_carry(H);
+  // -P negates t2: F and G swap.  D + 4q - C < 8q, D + C < 6q.
+  F Fm = neg ? f_add_nc(D, C) : f_sub_nc(D, C, K.q4);
+  F Gm = neg ? f_sub_nc(D, C, K.q4) : f_add_nc(D, C);
+  f_carry(Gm);
   a.x = f_mul(E, Fm);
   a.y = f_mul(Gm, H);
   a.t = f_mul(E, H);

@@ -9,9 +9,11 @@
 // reduction factor is m = -t0 (mod 2^28), no multiply.
 //
 // Bounds ("value < k q" is tracked at call sites):
-//   f_mul: inputs with limbs < 2^29 and values < 2^384 give a normalized
-//          output (limbs < 2^28) with value < 2q: the CIOS result is
-//          (a b + m q) / R' < a b / R' + q <= 2^768 / 2^392 + q < 2q.
+//   f_mul: inputs with values < 2^384 and limbs < 2^29 (or one input < 2^30
+//          and the other normalized, < 2^28) give a normalized output (limbs
+//          < 2^28) with value < 2q: each column sums <= 28 products < 2^58,
+//          and the CIOS result is (a b + m q) / R' < a b / R' + q
+//          <= 2^768 / 2^392 + q < 2q.
 //   fadd: normalized inputs, normalized output, value = a + b.
 //   fsub_k: a + k q - b, normalized; needs b < k q (k in {2, 4, 8}).
 //
@@ -75,6 +77,25 @@ ZP_HD void f_carry(F &a) {
     c = v >> 28;
   }
   a.l[13] += c;
+}
+
+// Limbwise a + b without carries: normalized inputs give limbs < 2^29.
+ZP_HD F f_add_nc(const F &a, const F &b) {
+  F r;
+#pragma unroll
+  for (int i = 0; i < 14; i++) r.l[i] = a.l[i] + b.l[i];
+  return r;
+}
+
+// Limbwise a + K q - b without carries (value a + K q - b).  Needs a with
+// limbs < 2^29, b normalized with b < K q / 2 so that every limb of the
+// redundant K q (see f_make_kq) covers b's limb, top limb included; gives
+// limbs < 2^30.
+ZP_HD F f_sub_nc(const F &a, const F &b, const uint32_t (&kq)[14]) {
+  F r;
+#pragma unroll
+  for (int i = 0; i < 14; i++) r.l[i] = a.l[i] + kq[i] - b.l[i];
+  return r;
 }
 
 ZP_HD F f_add(const F &a, const F &b) {

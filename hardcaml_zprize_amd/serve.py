@@ -178,6 +178,7 @@ class GpuServer:
         ntt_jobs = [j for j in batch if j.kind == NTT]
         msm_jobs = [j for j in batch if j.kind == MSM]
         held = []
+        keep = []  # host arrays that async copies read from, alive until the sync below
         sp = ctypes.c_void_p(stream)
         # inputs to HBM
         for j in ntt_jobs:
@@ -192,6 +193,7 @@ class GpuServer:
             p = pool.get(nb)
             held.append((nb, p))
             sc = np.ascontiguousarray(j.scalars, dtype=np.uint64)
+            keep.append(sc)
             call("zp_memcpy_h2d_async", ctypes.c_void_p(p), sc.ctypes.data_as(ctypes.c_void_p), sc.nbytes, sp)
             msm_ptrs.append(p)
         ntt_ptrs = [p for (nb, p) in held[:len(ntt_jobs)]]
