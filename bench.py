@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--msm-steps", type=int, default=3)
     ap.add_argument("--msm-warmup", type=int, default=1)
     ap.add_argument("--serve-jobs", type=int, default=120)
-    ap.add_argument("--serve-load", type=float, default=2.0)
+    ap.add_argument("--serve-load", type=float, default=6.0)
     ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--skip-msm", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")

@@ -43,9 +43,22 @@ __host__ __device__ __forceinline__ uint64_t gl_reduce128(uint64_t lo, uint64_t 
   return r >= GL_P ? r - GL_P : r;
 }
 
+// 64 x 64 -> 128-bit product from four 32 x 32 -> 64 multiply-adds (each
+// partial product plus one 32-bit carry word fits in 64 bits).
+__device__ __forceinline__ void mul_64x64(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p0 = (uint64_t)a0 * b0;
+  const uint64_t p1 = (uint64_t)a0 * b1 + (p0 >> 32);
+  const uint64_t p2 = (uint64_t)a1 * b0 + (uint32_t)p1;
+  const uint64_t p3 = (uint64_t)a1 * b1 + (p1 >> 32) + (p2 >> 32);
+  lo = (p2 << 32) | (uint32_t)p0;
+  hi = p3;
+}
+
 __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
-  uint64_t lo = a * b;
-  uint64_t hi = __umul64hi(a, b);
+  uint64_t lo, hi;
+  mul_64x64(a, b, lo, hi);
   return gl_reduce128(lo, hi);
 }
 

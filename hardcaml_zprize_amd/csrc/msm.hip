@@ -333,6 +333,10 @@ struct zp_msm_workspace {
   size_t cap_K = 0, cap_K2 = 0, cap_K3 = 0, cap_K4 = 0, cap_K5 = 0, cap_Kb = 0;
   size_t cap_entries = 0, cap_units = 0, cap_units2 = 0, cap_groups = 0, cap_gs = 0, cap_win = 0, cap_wsum = 0;
   size_t cap_jobs = 0, cap_pt0 = 0, cap_b0 = 0, cap_scan = 0;
+  size_t cap_digits = 0, cap_counts = 0, cap_scanned = 0, cap_segs = 0;
+  int32_t *digits = nullptr;
+  uint32_t *counts = nullptr, *scanned = nullptr;
+  SegDev *segs = nullptr;
   uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
   uint32_t *ubucket = nullptr, *vals = nullptr, *gstart = nullptr, *pt0 = nullptr, *b0 = nullptr;
   Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
@@ -342,8 +346,8 @@ struct zp_msm_workspace {
   std::vector<Xyzt> h_winsum;
 
   void release() {
-    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0,
-                  partial, buckets, groups, winsum, jobs, wins, scan_tmp};
+    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
+                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
@@ -454,7 +458,9 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   std::vector<MsmJobDev> jd(count);
   std::vector<WinDev> wins;
   std::vector<uint32_t> pt0(count), b0(count), gstart;
-  uint64_t K = 0, entries = 0, points = 0;
+  std::vector<SegDev> segs;
+  uint64_t K = 0, entries = 0, points = 0, counters = 0;
+  uint32_t max_nb = 0;
   for (size_t j = 0; j < count; j++) {
     const HostJob &h = jobs[j];
     MsmOptions o = h.ctx->opts;
@@ -488,11 +494,23 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       wd.base = (uint32_t)K;
       wd.top = w == L.W - 1;
       K += L.nb[w];
+      max_nb = std::max(max_nb, L.nb[w]);
       wins.push_back(wd);
     }
+    // sorted path: C chunks of <= chunk_len points per window
+    d.chunks = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, h.n >> 18));
+    d.chunk_len = (uint32_t)std::max<uint64_t>(1, (h.n + d.chunks - 1) / d.chunks);
+    d.dig0 = entries;
+    d.cnt0 = counters;
+    counters += (K - d.bucket0) * d.chunks;
+    for (int w = 0; w < L.W; w++)
+      for (uint32_t c = 0; c < d.chunks; c++) segs.push_back(SegDev{(uint32_t)j, d.win0 + (uint32_t)w, c});
     points += h.n;
     entries += (uint64_t)h.n * L.W;
   }
+  // The LDS histogram holds one window's buckets (<= 2^15 u32 = 128 KiB).
+  const char *force_atomic = getenv("ZP_MSM_ATOMIC_SORT");
+  const bool sorted_path = max_nb <= 32768 && !(force_atomic && atoi(force_atomic));
   ZP_CHECK(points < (1ull << 31) && K < (1ull << 31) && entries < (1ull << 32), ZP_ERR_TOO_LARGE,
            "MSM batch too large (%llu points, %llu buckets)", (unsigned long long)points, (unsigned long long)K);
   const uint32_t Wt = (uint32_t)wins.size();
@@ -523,15 +541,25 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_TRY(grow(ws.jobs, ws.cap_jobs, count));
   ZP_TRY(grow(ws.pt0, ws.cap_pt0, count));
   ZP_TRY(grow(ws.b0, ws.cap_b0, count));
-  size_t scan_bytes = 0;
+  size_t scan_bytes = 0, scan_bytes2 = 0;
   ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
                                  stream));
-  ZP_TRY(grow(ws.scan_tmp, ws.cap_scan, std::max<size_t>(scan_bytes, 16)));
+  if (sorted_path) {
+    ZP_TRY(grow(ws.digits, ws.cap_digits, std::max<uint64_t>(entries, 1)));
+    ZP_TRY(grow(ws.counts, ws.cap_counts, counters + 1));
+    ZP_TRY(grow(ws.scanned, ws.cap_scanned, counters + 1));
+    ZP_TRY(grow(ws.segs, ws.cap_segs, std::max<size_t>(segs.size(), 1)));
+    ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes2, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
+                                   rocprim::plus<uint32_t>(), stream));
+  }
+  ZP_TRY(grow(ws.scan_tmp, ws.cap_scan, std::max<size_t>(std::max(scan_bytes, scan_bytes2), 16)));
   ZP_HIP(hipMemcpyAsync(ws.jobs, jd.data(), count * sizeof(MsmJobDev), hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.wins, wins.data(), wins.size() * sizeof(WinDev), hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.pt0, pt0.data(), count * 4, hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.b0, b0.data(), count * 4, hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.gstart, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice, stream));
+  if (sorted_path && !segs.empty())
+    ZP_HIP(hipMemcpyAsync(ws.segs, segs.data(), segs.size() * sizeof(SegDev), hipMemcpyHostToDevice, stream));
   BatchDev B;
   B.jobs = ws.jobs;
   B.job_pt0 = ws.pt0;
@@ -541,27 +569,56 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   B.total_points = (uint32_t)points;
   B.K = (uint32_t)K;
   const FConsts &KC = consts();
-  // ---- kernels
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);
-  ZP_HIP(hipGetLastError());
-  if (points) {
-    hipLaunchKernelGGL(k_digits<0>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.cnt, nullptr, nullptr,
-                       nullptr);
+  // ---- kernels: counting sort of (bucket, point) entries
+  if (sorted_path) {
+    if (points) {
+      hipLaunchKernelGGL(k_digits_make, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.digits);
+      ZP_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_seg_count, dim3((unsigned)segs.size()), dim3(1024), max_nb * sizeof(uint32_t), stream, B,
+                         ws.segs, ws.digits, ws.counts);
+      ZP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, stream, ws.counts + counters, (uint64_t)1, 0u);
+    if (!points) {
+      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(counters + 1)), dim3(MSM_THREADS), 0, stream, ws.counts,
+                         (uint64_t)counters + 1, 0u);
+    }
     ZP_HIP(hipGetLastError());
-  }
-  size_t tb = ws.cap_scan;
-  ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
-                                 stream));
-  ZP_HIP(hipMemcpyAsync(ws.cursor, ws.off, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-  if (points) {
-    hipLaunchKernelGGL(k_digits<1>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, nullptr, ws.cursor,
-                       ws.vals, nullptr);
+    size_t tb0 = ws.cap_scan;
+    ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb0, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
+                                   rocprim::plus<uint32_t>(), stream));
+    hipLaunchKernelGGL(k_bucket_offsets, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, B, ws.scanned,
+                       (uint64_t)counters, ws.off);
+    hipLaunchKernelGGL(k_bucket_counts_from_off, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.off,
+                       (uint32_t)K, ws.cnt);
     ZP_HIP(hipGetLastError());
+    if (points) {
+      hipLaunchKernelGGL(k_seg_scatter, dim3((unsigned)segs.size()), dim3(1024), max_nb * sizeof(uint32_t), stream,
+                         B, ws.segs, ws.digits, ws.scanned, ws.vals);
+      ZP_HIP(hipGetLastError());
+    }
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);
+    ZP_HIP(hipGetLastError());
+    if (points) {
+      hipLaunchKernelGGL(k_digits<0>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.cnt, nullptr,
+                         nullptr, nullptr);
+      ZP_HIP(hipGetLastError());
+    }
+    size_t tb = ws.cap_scan;
+    ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
+                                   stream));
+    ZP_HIP(hipMemcpyAsync(ws.cursor, ws.off, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    if (points) {
+      hipLaunchKernelGGL(k_digits<1>, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, nullptr, ws.cursor,
+                         ws.vals, nullptr);
+      ZP_HIP(hipGetLastError());
+    }
   }
   hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint32_t)K,
                      log_u, ws.nunits);
   ZP_HIP(hipGetLastError());
-  tb = ws.cap_scan;
+  size_t tb = ws.cap_scan;
   ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, (size_t)K + 1,
                                  rocprim::plus<uint32_t>(), stream));
   hipLaunchKernelGGL(k_fill_units, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.nunits, ws.ustart,

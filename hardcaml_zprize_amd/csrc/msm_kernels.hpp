@@ -38,6 +38,15 @@ struct MsmJobDev {
   uint32_t pt0;             // first global point slot of this job
   uint32_t win0, W;         // windows [win0, win0 + W)
   uint32_t bucket0;         // first global bucket of this job
+  uint32_t chunks;          // point chunks of the sorted path (C)
+  uint32_t chunk_len;       // points per chunk
+  uint64_t dig0;            // first digit slot: digit(w, i) at dig0 + w * n + i
+  uint64_t cnt0;            // first per-(bucket, chunk) counter: cnt0 + b_local * C + c
+};
+
+// One block of the sorted path: window w (global) of job j, chunk c.
+struct SegDev {
+  uint32_t job, win, chunk;
 };
 
 struct WinDev {
@@ -138,6 +147,84 @@ __global__ void __launch_bounds__(MSM_THREADS) k_digits(BatchDev B, uint32_t *cn
       }
     }
   }
+}
+
+// ---- sorted path: digits once, then per-(window, chunk) LDS histograms ----
+// Signed digits of every (job, point, window), zero for points without a
+// twisted-Edwards image (their scalars are handled by the host fallback).
+__global__ void __launch_bounds__(MSM_THREADS) k_digits_make(BatchDev B, int32_t *digits) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < B.total_points; g += gridDim.x * blockDim.x) {
+    const uint32_t j = upper_index(B.job_pt0, B.J, g);
+    const MsmJobDev jb = B.jobs[j];
+    const uint32_t i = g - jb.pt0;
+    int32_t *dst = digits + jb.dig0 + i;
+    if (!jb.conv[i]) {
+      for (uint32_t w = 0; w < jb.W; w++) dst[(uint64_t)w * jb.n] = 0;
+      continue;
+    }
+    uint64_t s[4];
+    load_scalar(jb.scalars, i, s);
+    uint32_t carry = 0;
+    for (uint32_t w = 0; w < jb.W; w++) dst[(uint64_t)w * jb.n] = window_digit(s, B.wins[jb.win0 + w], carry);
+  }
+}
+
+// Per segment (job, window, chunk): LDS histogram of the chunk's digits,
+// written to counts[cnt0 + b * C + c] (every bucket, zeros included).
+__global__ void __launch_bounds__(1024) k_seg_count(BatchDev B, const SegDev *segs, const int32_t *digits,
+                                                    uint32_t *counts) {
+  extern __shared__ uint32_t hist[];
+  const SegDev sg = segs[blockIdx.x];
+  const MsmJobDev jb = B.jobs[sg.job];
+  const WinDev wd = B.wins[sg.win];
+  for (uint32_t b = threadIdx.x; b < wd.nb; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
+  const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int32_t d = dig[i];
+    if (d != 0) atomicAdd(&hist[(uint32_t)(d < 0 ? -d : d) - 1], 1u);
+  }
+  __syncthreads();
+  uint32_t *dst = counts + jb.cnt0 + (uint64_t)(wd.base - jb.bucket0) * jb.chunks + sg.chunk;
+  for (uint32_t b = threadIdx.x; b < wd.nb; b += blockDim.x) dst[(uint64_t)b * jb.chunks] = hist[b];
+}
+
+// Per segment: cursors from the scanned counts, then LDS-atomic placement.
+__global__ void __launch_bounds__(1024) k_seg_scatter(BatchDev B, const SegDev *segs, const int32_t *digits,
+                                                      const uint32_t *scanned, uint32_t *vals) {
+  extern __shared__ uint32_t cursor[];
+  const SegDev sg = segs[blockIdx.x];
+  const MsmJobDev jb = B.jobs[sg.job];
+  const WinDev wd = B.wins[sg.win];
+  const uint32_t *src = scanned + jb.cnt0 + (uint64_t)(wd.base - jb.bucket0) * jb.chunks + sg.chunk;
+  for (uint32_t b = threadIdx.x; b < wd.nb; b += blockDim.x) cursor[b] = src[(uint64_t)b * jb.chunks];
+  __syncthreads();
+  const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
+  const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int32_t d = dig[i];
+    if (d == 0) continue;
+    const uint32_t pos = atomicAdd(&cursor[(uint32_t)(d < 0 ? -d : d) - 1], 1u);
+    vals[pos] = i | (d < 0 ? 0x80000000u : 0u);
+  }
+}
+
+// off[k] = scanned[cnt0 + b * C] for bucket k (local b of its job); off[K] = total.
+__global__ void k_bucket_offsets(BatchDev B, const uint32_t *scanned, uint64_t total_counters, uint32_t *off) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= B.K; k += gridDim.x * blockDim.x) {
+    if (k == B.K) {
+      off[k] = scanned[total_counters];
+      continue;
+    }
+    const MsmJobDev jb = B.jobs[upper_index(B.job_b0, B.J, k)];
+    off[k] = scanned[jb.cnt0 + (uint64_t)(k - jb.bucket0) * jb.chunks];
+  }
+}
+
+__global__ void k_bucket_counts_from_off(const uint32_t *off, uint32_t K, uint32_t *cnt) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x)
+    cnt[k] = off[k + 1] - off[k];
 }
 
 // nunits[k] = ceil(cnt[k] / U); k == K gets 0 (so the scan's last entry is the total).

@@ -117,15 +117,15 @@ static NttShape shape_of(unsigned log_n) {
   return s;
 }
 
-static int g_ntt_tile = -1;  // T of the 4096-point kernels (ZP_NTT_TILE=4|8)
+// Tile widths (sequences per workgroup) of the 4096-point kernels, per pass:
+// ZP_NTT_TILE_COLS / ZP_NTT_TILE_ROWS = 4 or 8 (ZP_NTT_TILE sets both).
+static int g_tile_cols = -1, g_tile_rows = -1;
 
-static int ntt_tile() {
-  if (g_ntt_tile < 0) {
-    const char *e = getenv("ZP_NTT_TILE");
-    int v = e ? atoi(e) : 8;
-    g_ntt_tile = (v == 4) ? 4 : 8;
-  }
-  return g_ntt_tile;
+static int env_tile(const char *name, int dflt) {
+  const char *e = getenv(name);
+  if (!e) e = getenv("ZP_NTT_TILE");
+  const int v = e ? atoi(e) : dflt;
+  return v == 4 ? 4 : (v == 8 ? 8 : dflt);
 }
 
 // Job tables for the batched form: data_tab[j] / scratch_tab[j] (device arrays), jobs = grid.y.
@@ -136,51 +136,66 @@ struct JobTabs {
 };
 
 template <int T>
-static int launch_4096_pair(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, hipStream_t stream,
+static int launch_4096_cols(const NttTables *t, uint64_t *d_in, uint64_t *d_out, int dir, hipStream_t stream,
                             const JobTabs &jt) {
-  const uint64_t n1 = 4096, n2 = 4096;
   Ntt4096Args a{};
-  a.n1 = n1;
-  a.n2 = n2;
+  a.n1 = 4096;
+  a.n2 = 4096;
   a.tw4096 = t->d_4096[dir];
   a.tw_lo = t->d_lo[dir];
   a.tw_hi = t->d_hi[dir];
-  a.tiles = (uint32_t)(n2 / T);
-  a.in = d_data;
-  a.out = d_scratch;
+  a.tiles = 4096 / T;
+  a.in = d_in;
+  a.out = d_out;
   a.in_tab = jt.data;
   a.out_tab = jt.scratch;
   a.scale = 1;
-  const size_t lds = ntt4096_lds_bytes<T>();
-  const dim3 grid1(a.tiles, jt.jobs);
+  const dim3 grid(a.tiles, jt.jobs);
   if (dir)
-    hipLaunchKernelGGL((k_ntt4096_cols<T, 1>), grid1, dim3(64 * T), lds, stream, a);
+    hipLaunchKernelGGL((k_ntt4096_cols<T, 1>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
   else
-    hipLaunchKernelGGL((k_ntt4096_cols<T, 0>), grid1, dim3(64 * T), lds, stream, a);
+    hipLaunchKernelGGL((k_ntt4096_cols<T, 0>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
   ZP_HIP(hipGetLastError());
-  a.in = d_scratch;
-  a.out = d_data;
+  return ZP_OK;
+}
+
+template <int T>
+static int launch_4096_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out, int dir, hipStream_t stream,
+                            const JobTabs &jt) {
+  Ntt4096Args a{};
+  a.n1 = 4096;
+  a.n2 = 4096;
+  a.tw4096 = t->d_4096[dir];
+  a.tiles = 4096 / T;
+  a.in = d_in;
+  a.out = d_out;
   a.in_tab = jt.scratch;
   a.out_tab = jt.data;
   a.scale = dir ? t->n_inv : 1;
-  a.tiles = (uint32_t)(n1 / T);
-  const dim3 grid2(a.tiles, jt.jobs);
+  const dim3 grid(a.tiles, jt.jobs);
   if (dir)
-    hipLaunchKernelGGL((k_ntt4096_rows<T, 1>), grid2, dim3(64 * T), lds, stream, a);
+    hipLaunchKernelGGL((k_ntt4096_rows<T, 1>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
   else
-    hipLaunchKernelGGL((k_ntt4096_rows<T, 0>), grid2, dim3(64 * T), lds, stream, a);
+    hipLaunchKernelGGL((k_ntt4096_rows<T, 0>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
   ZP_HIP(hipGetLastError());
   return ZP_OK;
+}
+
+static int launch_4096_pair(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, hipStream_t stream,
+                            const JobTabs &jt) {
+  if (g_tile_cols < 0) g_tile_cols = env_tile("ZP_NTT_TILE_COLS", 8);
+  if (g_tile_rows < 0) g_tile_rows = env_tile("ZP_NTT_TILE_ROWS", 8);
+  ZP_TRY(g_tile_cols == 4 ? launch_4096_cols<4>(t, d_data, d_scratch, dir, stream, jt)
+                          : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt));
+  return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
+                          : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
 }
 
 // Enqueue one in-place transform.  scratch must hold n elements when log_n > 12.
 static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir,
                             hipStream_t stream, const JobTabs &jt = JobTabs()) {
   const NttShape sh = shape_of(t->log_n);
-  if (sh.log_n1 == 12 && sh.log_n2 == 12) {
-    return ntt_tile() == 4 ? launch_4096_pair<4>(t, d_data, d_scratch, dir, stream, jt)
-                           : launch_4096_pair<8>(t, d_data, d_scratch, dir, stream, jt);
-  }
+  if (sh.log_n1 == 12 && sh.log_n2 == 12) return launch_4096_pair(t, d_data, d_scratch, dir, stream, jt);
   const uint64_t n = 1ull << sh.log_n;
   const uint64_t scale = dir ? t->n_inv : 1;
   if (sh.log_n2 == 0) {

@@ -68,10 +68,17 @@ __device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
       constexpr int i0 = (b / half) * 2 * half + j;
       constexpr int e0 = (39 * j * (64 >> s)) % 192;
       constexpr int e = DIR ? (192 - e0) % 192 : e0;
+      // 2^e = -2^(e - 96) for e >= 96: the sign swaps the butterfly's add and sub.
       const uint64_t u = y[i0];
-      const uint64_t t = gl_shl<e>(y[i0 + half]);
-      y[i0] = gl_add(u, t);
-      y[i0 + half] = gl_sub(u, t);
+      if constexpr (e >= 96) {
+        const uint64_t t = gl_shl<e - 96>(y[i0 + half]);
+        y[i0] = gl_sub(u, t);
+        y[i0 + half] = gl_add(u, t);
+      } else {
+        const uint64_t t = gl_shl<e>(y[i0 + half]);
+        y[i0] = gl_add(u, t);
+        y[i0 + half] = gl_sub(u, t);
+      }
     });
   });
   static_for<64>([&](auto i) { x[i] = y[i]; });

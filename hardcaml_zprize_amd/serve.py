@@ -20,6 +20,7 @@ from collections import deque
 
 import numpy as np
 
+from . import autotune
 from ._lib import call, lib
 from .msm import G1_PROJ_DTYPE, MsmWorkspace
 from .ntt import NttBatch
@@ -95,6 +96,7 @@ class GpuServer:
         self.cv = threading.Condition()
         self.stop = False
         self.batches = 0
+        self.windows = autotune.load()  # size class -> tuned MSM window (empty = library default)
         self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
         for t in self.threads:
             t.start()
@@ -202,7 +204,9 @@ class GpuServer:
             b = NttBatch(ntt_ptrs, [j.log_n for j in ntt_jobs], [j.direction for j in ntt_jobs])
             b.launch(use_graph=False, stream=stream)
         if msm_jobs:
-            specs = [(self.tables[j.table], len(j.scalars), p, j.window) for j, p in zip(msm_jobs, msm_ptrs)]
+            specs = [(self.tables[j.table], len(j.scalars), p,
+                      j.window or self.windows.get(max(len(j.scalars) - 1, 1).bit_length(), 0))
+                     for j, p in zip(msm_jobs, msm_ptrs)]
             out = ws.run(specs, stream=stream)
             for j, o in zip(msm_jobs, out):
                 j.result = o
