@@ -45,7 +45,7 @@ __host__ __device__ __forceinline__ uint64_t gl_reduce128(uint64_t lo, uint64_t 
 
 // 64 x 64 -> 128-bit product from four 32 x 32 -> 64 multiply-adds (each
 // partial product plus one 32-bit carry word fits in 64 bits).
-__device__ __forceinline__ void mul_64x64(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+__host__ __device__ __forceinline__ void mul_64x64(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
   const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
   const uint64_t p0 = (uint64_t)a0 * b0;
@@ -82,6 +82,51 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t a, unsigned s) {
     r = gl_reduce128(0, b);
   }
   return negate ? gl_neg(r) : r;
+}
+
+// ---- lazy forms: operands and results are any u64 standing for its residue
+// mod p; only the arguments named "t" below must be canonical.  They let the
+// 4096-point kernels skip the final compare-and-subtract on most values and
+// canonicalize once, where a bound needs it or before a value is stored.
+
+__host__ __device__ __forceinline__ uint64_t gl_canon(uint64_t a) { return a >= GL_P ? a - GL_P : a; }
+
+// u + t, t < p.  u + t < 2^64 + p, so after a carry s < p - 1 and s + eps
+// cannot carry again.
+__host__ __device__ __forceinline__ uint64_t gl_add_lz(uint64_t u, uint64_t t) {
+  const uint64_t s = u + t;
+  const uint32_t c = s < t;
+  return s + (uint64_t)c * GL_EPS;
+}
+
+// u - t, t < p.  After a borrow d = u - t + 2^64 >= 2^64 - p + 1 = eps, so
+// d - eps (the 2^64 = eps fold) cannot borrow again.
+__host__ __device__ __forceinline__ uint64_t gl_sub_lz(uint64_t u, uint64_t t) {
+  const uint64_t d = u - t;
+  const uint32_t b = u < t;
+  return d - (uint64_t)b * GL_EPS;
+}
+
+// hi * 2^64 + lo reduced to a u64 of the same residue (not canonical):
+// lo + hl * eps - hh with hi = hh 2^32 + hl (2^64 = eps, 2^96 = -1).
+__host__ __device__ __forceinline__ uint64_t gl_reduce128_lz(uint64_t lo, uint64_t hi) {
+  const uint32_t hl = (uint32_t)hi, hh = (uint32_t)(hi >> 32);
+  const uint64_t r0 = (uint64_t)hl * GL_EPS + lo;
+  const bool c = r0 < lo;  // r0 stands for r0 + 2^64 = r0 + eps
+  const uint64_t r1 = r0 - hh;
+  const bool b = r0 < hh;  // r1 stands for r1 - 2^64 = r1 - eps
+  // c only: r0 < (2^32 - 1)^2, so r1 + eps < 2^64.  b only: r1 >= 2^64 - 2^32,
+  // so r1 - eps >= 0.  Both or neither: r1 as it is.
+  uint64_t r = r1;
+  if (c && !b) r = r1 + GL_EPS;
+  if (b && !c) r = r1 - GL_EPS;
+  return r;
+}
+
+__host__ __device__ __forceinline__ uint64_t gl_mul_lz(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul_64x64(a, b, lo, hi);
+  return gl_reduce128_lz(lo, hi);
 }
 
 __host__ __forceinline__ uint64_t gl_mul_host(uint64_t a, uint64_t b) {

@@ -45,7 +45,7 @@ struct NttTables {
   uint64_t *d_sub[2] = {nullptr, nullptr};  // w_4096^i, i < 2048 (dir 0 / dir 1)
   uint64_t *d_lo[2] = {nullptr, nullptr};   // w_n^i, i < 4096
   uint64_t *d_hi[2] = {nullptr, nullptr};   // w_n^(4096 i), i < max(1, n / 4096)
-  uint64_t *d_4096[2] = {nullptr, nullptr}; // w_4096^i, i < 4096
+  uint64_t *d_4096[2] = {nullptr, nullptr}; // [k][a] = w_4096^(a k), a, k < 64
   uint64_t n_inv = 1;
 };
 
@@ -66,7 +66,7 @@ static int make_tables(unsigned log_n, NttTables **out) {
   t->log_n = log_n;
   const uint64_t n = 1ull << log_n;
   const uint64_t n_hi = std::max<uint64_t>(1, n >> 12);
-  std::vector<uint64_t> sub(2048), lo(4096), hi(n_hi), full(4096);
+  std::vector<uint64_t> sub(2048), lo(4096), hi(n_hi), full(4096), ka(4096);
   for (int dir = 0; dir < 2; dir++) {
     uint64_t w4096 = gl_root_host(12);
     uint64_t wn = gl_root_host(log_n);
@@ -86,7 +86,9 @@ static int make_tables(unsigned log_n, NttTables **out) {
     for (uint64_t i = 0; i < n_hi; i++, acc = gl_mul_host(acc, step)) hi[i] = acc;
     ZP_HIP(hipMalloc(&t->d_sub[dir], 2048 * 8));
     ZP_HIP(hipMalloc(&t->d_4096[dir], 4096 * 8));
-    ZP_HIP(hipMemcpy(t->d_4096[dir], full.data(), 4096 * 8, hipMemcpyHostToDevice));
+    for (int k = 0; k < 64; k++)
+      for (int a = 0; a < 64; a++) ka[k * 64 + a] = full[a * k];
+    ZP_HIP(hipMemcpy(t->d_4096[dir], ka.data(), 4096 * 8, hipMemcpyHostToDevice));
     ZP_HIP(hipMalloc(&t->d_lo[dir], 4096 * 8));
     ZP_HIP(hipMalloc(&t->d_hi[dir], n_hi * 8));
     ZP_HIP(hipMemcpy(t->d_sub[dir], sub.data(), 2048 * 8, hipMemcpyHostToDevice));

@@ -4,7 +4,8 @@
 // thread (s, a) keeps the 64 elements j = a + 64 m (m = 0..63) of sequence s.
 //   step A  radix-64 DFT over m in registers; every twiddle is a power of two
 //           (w_64 = OMEGA[6] = 2^39), so it is a shift and a reduction;
-//   step B  twiddle w_4096^(a k') (table in global memory, L1/L2 resident);
+//   step B  twiddle w_4096^(a k') (table [k'][a] in global memory, L1/L2
+//           resident: the lanes of a wave read one contiguous run);
 //   step C  transpose through LDS (32-bit halves, 2 rounds) so thread (s, k')
 //           holds the 64 values over a;
 //   step D  radix-64 DFT over a -> output k = k' + 64 k''.
@@ -39,13 +40,53 @@ __device__ __forceinline__ uint64_t gl_shl(uint64_t a) {
   }
 }
 
+// t = v * 2^E, canonical (< p), for any u64 v and a compile-time E in [1, 96).
+// Three closed forms, by where the product's 2^64 and 2^96 words fall:
+//   E <= 32      v 2^E = lo + hi eps, hi = v >> (64 - E) < 2^32 (2^64 = eps);
+//                after a carry the result is already < p.
+//   32 < E < 64  v 2^(E-32) = lo + hi 2^64, so v 2^E = lo 2^32 - hi
+//                = (lo_lo + lo_hi) 2^32 - (lo_hi + hi) = A 2^32 - B
+//                = A0 2^32 + A1 eps - B  (A, B < 2^33), in (-2^33, p): add p if
+//                negative.
+//   E >= 64      v 2^E = -v 2^-K (K = 96 - E in (0, 32]) = M eps - q with
+//                q = v >> K and M = low word of v << (32 - K); M eps < p.
+template <int E>
+__host__ __device__ __forceinline__ uint64_t gl_shl_lz(uint64_t v) {
+  static_assert(E > 0 && E < 96, "exponent class");
+  const uint32_t vl = (uint32_t)v, vh = (uint32_t)(v >> 32);
+  if constexpr (E <= 32) {
+    const uint64_t lo = v << E;
+    const uint32_t hi = (uint32_t)(v >> (64 - E));
+    const uint64_t r0 = (uint64_t)hi * GL_EPS + lo;
+    const bool c = r0 < lo;
+    return (c || r0 >= GL_P) ? r0 + GL_EPS : r0;
+  } else if constexpr (E < 64) {
+    constexpr int F = E - 32;
+    const uint32_t lo_lo = vl << F;
+    const uint32_t lo_hi = (uint32_t)(v >> (32 - F));
+    const uint32_t hi = vh >> (32 - F);
+    const uint64_t A = (uint64_t)lo_lo + lo_hi;
+    const uint64_t B = (uint64_t)lo_hi + hi;
+    const uint64_t Y = (uint64_t)(uint32_t)(A >> 32) * GL_EPS + (A << 32);
+    const uint64_t d = Y - B;
+    return Y < B ? d + GL_P : d;
+  } else {
+    constexpr int K = 96 - E;
+    const uint32_t M = K == 32 ? vl : (vl << ((32 - K) & 31));
+    const uint64_t q = v >> K;
+    const uint64_t me = (uint64_t)M * GL_EPS;
+    const uint64_t d = me - q;
+    return me < q ? d + GL_P : d;
+  }
+}
+
 // Compile-time loop: f(std::integral_constant<int, I>) for I in [0, N).
 template <typename Fn, int... I>
-__device__ __forceinline__ void static_for_impl(Fn &&f, std::integer_sequence<int, I...>) {
+__host__ __device__ __forceinline__ void static_for_impl(Fn &&f, std::integer_sequence<int, I...>) {
   (f(std::integral_constant<int, I>{}), ...);
 }
 template <int N, typename Fn>
-__device__ __forceinline__ void static_for(Fn &&f) {
+__host__ __device__ __forceinline__ void static_for(Fn &&f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
@@ -56,8 +97,12 @@ constexpr int bitrev6(int i) {
 // y[k] = sum_m x[m] w_64^(m k) in place, natural order in and out: radix-2
 // DIT on the bit-reversed input, every index a compile-time constant so the
 // 64 values stay in registers.  DIR 0: w_64 = 2^39; DIR 1: w_64^-1 = 2^153.
+// Lazy: x[32..63] must be canonical (they are the twiddled operands of the
+// first stage), x[0..31] may be any u64; the results are any u64.  Every
+// later twiddled operand is made canonical by gl_shl_lz or gl_canon, which is
+// what gl_add_lz / gl_sub_lz need.
 template <int DIR>
-__device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
+__host__ __device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
   uint64_t y[64];
   static_for<64>([&](auto i) { y[i] = x[bitrev6(i)]; });
   static_for<6>([&](auto sm1) {
@@ -69,26 +114,62 @@ __device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
       constexpr int e0 = (39 * j * (64 >> s)) % 192;
       constexpr int e = DIR ? (192 - e0) % 192 : e0;
       // 2^e = -2^(e - 96) for e >= 96: the sign swaps the butterfly's add and sub.
+      constexpr int em = e % 96;
       const uint64_t u = y[i0];
-      if constexpr (e >= 96) {
-        const uint64_t t = gl_shl<e - 96>(y[i0 + half]);
-        y[i0] = gl_sub(u, t);
-        y[i0 + half] = gl_add(u, t);
+      uint64_t t;
+      if constexpr (em == 0) {
+        t = s == 1 ? y[i0 + half] : gl_canon(y[i0 + half]);
       } else {
-        const uint64_t t = gl_shl<e>(y[i0 + half]);
-        y[i0] = gl_add(u, t);
-        y[i0 + half] = gl_sub(u, t);
+        t = gl_shl_lz<em>(y[i0 + half]);
+      }
+      if constexpr (e >= 96) {
+        y[i0] = gl_sub_lz(u, t);
+        y[i0 + half] = gl_add_lz(u, t);
+      } else {
+        y[i0] = gl_add_lz(u, t);
+        y[i0 + half] = gl_sub_lz(u, t);
       }
     });
   });
   static_for<64>([&](auto i) { x[i] = y[i]; });
 }
 
+// x[32..63] made canonical: the precondition of dft64_regs.
+__host__ __device__ __forceinline__ void canon_upper(uint64_t (&x)[64]) {
+  static_for<32>([&](auto i) { x[32 + i] = gl_canon(x[32 + i]); });
+}
+
+// base[off] for a wave-uniform base and a per-lane element offset < 2^29:
+// the address is an SGPR pair plus one 32-bit VGPR, so unrolled loops over
+// a compile-time row index keep no per-row 64-bit addresses live.
+__device__ __forceinline__ uint64_t ld_u64(const uint64_t *base, uint32_t off) {
+  return *(const uint64_t *)((const char *)base + (off << 3));
+}
+__device__ __forceinline__ void st_u64(uint64_t *base, uint32_t off, uint64_t v) {
+  *(uint64_t *)((char *)base + (off << 3)) = v;
+}
+
+// x[k] *= w_4096^(a k) for k = 1..63, tw = table + a.  Loaded eight at a
+// time, with scheduling barriers, so the compiler does not hoist all 63 loads
+// (126 VGPRs) above the multiplies.
+__device__ __forceinline__ void inner_twiddle(uint64_t (&x)[64], const uint64_t *tw) {
+  static_for<8>([&](auto g) {
+    uint64_t w[8];
+    static_for<8>([&](auto i) {
+      if constexpr (g * 8 + i > 0) w[i] = tw[(g * 8 + i) * 64];
+    });
+    static_for<8>([&](auto i) {
+      if constexpr (g * 8 + i > 0) x[g * 8 + i] = gl_mul_lz(x[g * 8 + i], w[i]);
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
 struct Ntt4096Args {
   const uint64_t *in;
   uint64_t *out;
   uint64_t n1, n2;         // matrix rows / columns (n = n1 * n2)
-  const uint64_t *tw4096;  // w_4096^i, i < 4096 (direction-specific)
+  const uint64_t *tw4096;  // [k][a] = w_4096^(a k), a, k < 64 (direction-specific)
   const uint64_t *tw_lo;   // w_n^i, i < 4096 (pass 1 only)
   const uint64_t *tw_hi;   // w_n^(4096 i)
   uint64_t scale;          // pass 2: n^-1 for the inverse, else 1
@@ -140,8 +221,17 @@ __device__ __forceinline__ void exchange(uint64_t (&x)[64], uint32_t *lds, uint3
   for (int a = 0; a < 64; a++) x[a] = ((uint64_t)lds[xidx<T, ROWS>(kp_r, a, s_r)] << 32) | lo[a];
 }
 
+// A 4-wave tile (T = 4) is held to 256 VGPRs so that two workgroups share a
+// CU (LDS 66.5 KB each) and one's loads and barriers overlap the other's
+// arithmetic; an 8-wave tile fills the CU by itself.
+template <int T>
+constexpr int ntt4096_waves_per_eu() {
+  return T <= 4 ? 2 : 1;
+}
+
 template <int T, int DIR>
-__global__ void __launch_bounds__(64 * T) k_ntt4096_cols(Ntt4096Args args) {
+__global__ void __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(ntt4096_waves_per_eu<T>())))
+k_ntt4096_cols(Ntt4096Args args) {
   extern __shared__ uint32_t lds[];
   if (args.in_tab) {
     args.in = args.in_tab[blockIdx.y];
@@ -152,16 +242,20 @@ __global__ void __launch_bounds__(64 * T) k_ntt4096_cols(Ntt4096Args args) {
   const uint32_t s = tid % T, a = tid / T;  // column within tile, sub-sequence
   const uint64_t col = (uint64_t)tile * T + s;
   const uint64_t n2 = args.n2;
+  const uint32_t lane = a * (uint32_t)n2 + s;  // element (a, s) of the tile's first row block
   uint64_t x[64];
 #pragma unroll
-  for (int m = 0; m < 64; m++) x[m] = args.in[(uint64_t)(a + 64 * m) * n2 + col];
+  for (int m = 0; m < 64; m++) x[m] = ld_u64(args.in + (uint64_t)(64 * m) * n2 + (uint64_t)tile * T, lane);
+  canon_upper(x);  // inputs >= p count as their residue
   dft64_regs<DIR>(x);
-#pragma unroll
-  for (int k = 1; k < 64; k++) x[k] = gl_mul(x[k], args.tw4096[a * k]);
+  inner_twiddle(x, args.tw4096 + a);
   exchange<T, false>(x, lds, s, a, s, a);
+  canon_upper(x);
   dft64_regs<DIR>(x);
   // x[k''] = output k1 = a + 64 k'' (a now plays k').  4-step twiddle
-  // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.
+  // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.  The scratch matrix
+  // holds lazy (not necessarily canonical) values; pass 2 canonicalizes the
+  // ones its first butterfly stage needs.
   const uint64_t n = args.n1 * n2;
   const uint64_t e0 = (col * a) & (n - 1);
   const uint64_t e1 = (col * 64) & (n - 1);
@@ -169,13 +263,14 @@ __global__ void __launch_bounds__(64 * T) k_ntt4096_cols(Ntt4096Args args) {
   const uint64_t step = gl_mul(args.tw_hi[e1 >> 12], args.tw_lo[e1 & 4095]);
 #pragma unroll
   for (int k = 0; k < 64; k++) {
-    args.out[(uint64_t)(a + 64 * k) * n2 + col] = gl_mul(x[k], w);
-    w = gl_mul(w, step);
+    st_u64(args.out + (uint64_t)(64 * k) * n2 + (uint64_t)tile * T, lane, gl_mul_lz(x[k], w));
+    w = gl_mul_lz(w, step);
   }
 }
 
 template <int T, int DIR>
-__global__ void __launch_bounds__(64 * T) k_ntt4096_rows(Ntt4096Args args) {
+__global__ void __launch_bounds__(64 * T) __attribute__((amdgpu_waves_per_eu(ntt4096_waves_per_eu<T>())))
+k_ntt4096_rows(Ntt4096Args args) {
   extern __shared__ uint32_t lds[];
   if (args.in_tab) {
     args.in = args.in_tab[blockIdx.y];
@@ -187,23 +282,25 @@ __global__ void __launch_bounds__(64 * T) k_ntt4096_rows(Ntt4096Args args) {
   const uint32_t a = tid & 63, r = tid >> 6;
   const uint64_t row0 = (uint64_t)tile * T;
   const uint64_t n2 = args.n2;
-  const uint64_t *src = args.in + (row0 + r) * n2;
+  const uint32_t lane_in = r * (uint32_t)n2 + a;
   uint64_t x[64];
 #pragma unroll
-  for (int m = 0; m < 64; m++) x[m] = src[a + 64 * m];
+  for (int m = 0; m < 64; m++) x[m] = ld_u64(args.in + row0 * n2 + 64 * m, lane_in);
+  canon_upper(x);
   dft64_regs<DIR>(x);
-#pragma unroll
-  for (int k = 1; k < 64; k++) x[k] = gl_mul(x[k], args.tw4096[a * k]);
+  inner_twiddle(x, args.tw4096 + a);
   // after the exchange thread (r2, kp) with r2 fastest, for T-row store runs
   const uint32_t r2 = tid % T, kp = tid / T;
   exchange<T, true>(x, lds, r, a, r2, kp);
+  canon_upper(x);
   dft64_regs<DIR>(x);
   const uint64_t n1 = args.n1;
+  const uint32_t lane_out = kp * (uint32_t)n1 + r2;
 #pragma unroll
   for (int k = 0; k < 64; k++) {
     uint64_t v = x[k];
-    if (args.scale != 1) v = gl_mul(v, args.scale);
-    args.out[(uint64_t)(kp + 64 * k) * n1 + row0 + r2] = v;
+    v = gl_canon(args.scale != 1 ? gl_mul_lz(v, args.scale) : v);
+    st_u64(args.out + (uint64_t)(64 * k) * n1 + row0, lane_out, v);
   }
 }
 
