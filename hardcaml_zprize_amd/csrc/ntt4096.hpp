@@ -94,59 +94,88 @@ constexpr int bitrev6(int i) {
   return ((i & 1) << 5) | ((i & 2) << 3) | ((i & 4) << 1) | ((i & 8) >> 1) | ((i & 16) >> 3) | ((i & 32) >> 5);
 }
 
+constexpr int bitrev3(int i) { return ((i & 1) << 2) | (i & 2) | ((i & 4) >> 2); }
+
+// Butterfly B (0..31) of radix-2 DIT stage S (half-size 2^(S-1)) on the
+// bit-reversed array y, lazy: the twiddled operand is made canonical (by
+// gl_shl_lz, or gl_canon for a unit twiddle after stage 1), which is what
+// gl_add_lz / gl_sub_lz need; the other operand and the results are any u64.
+template <int DIR, int S, int B>
+__host__ __device__ __forceinline__ void dit_bfly(uint64_t (&y)[64]) {
+  constexpr int half = 1 << (S - 1);
+  constexpr int j = B % half;
+  constexpr int i0 = (B / half) * 2 * half + j;
+  constexpr int e0 = (39 * j * (64 >> S)) % 192;
+  constexpr int e = DIR ? (192 - e0) % 192 : e0;
+  // 2^e = -2^(e - 96) for e >= 96: the sign swaps the butterfly's add and sub.
+  constexpr int em = e % 96;
+  const uint64_t u = y[i0];
+  uint64_t t;
+  if constexpr (em == 0) {
+    t = S == 1 ? y[i0 + half] : gl_canon(y[i0 + half]);
+  } else {
+    t = gl_shl_lz<em>(y[i0 + half]);
+  }
+  if constexpr (e >= 96) {
+    y[i0] = gl_sub_lz(u, t);
+    y[i0 + half] = gl_add_lz(u, t);
+  } else {
+    y[i0] = gl_add_lz(u, t);
+    y[i0 + half] = gl_sub_lz(u, t);
+  }
+}
+
 // y[k] = sum_m x[m] w_64^(m k) in place, natural order in and out: radix-2
 // DIT on the bit-reversed input, every index a compile-time constant so the
 // 64 values stay in registers.  DIR 0: w_64 = 2^39; DIR 1: w_64^-1 = 2^153.
-// Lazy: x[32..63] must be canonical (they are the twiddled operands of the
-// first stage), x[0..31] may be any u64; the results are any u64.  Every
-// later twiddled operand is made canonical by gl_shl_lz or gl_canon, which is
-// what gl_add_lz / gl_sub_lz need.
-template <int DIR>
+// Lazy: x[32..63] are the twiddled operands of the first stage and must be
+// canonical -- CANON_IN makes them so here, class by class -- x[0..31] may be
+// any u64; the results are any u64.
+// Stages 1-3 only combine the eight values x[c + 8u] (u < 8) of one residue
+// class c mod 8, which sit at y[8 bitrev3(c) .. + 8): they run class by class,
+// c = 0..7, so when the caller has issued its loads in the same order
+// (load_order) the first classes are transformed while later loads are still
+// in flight.
+template <int DIR, bool CANON_IN = false>
 __host__ __device__ __forceinline__ void dft64_regs(uint64_t (&x)[64]) {
   uint64_t y[64];
   static_for<64>([&](auto i) { y[i] = x[bitrev6(i)]; });
-  static_for<6>([&](auto sm1) {
-    constexpr int s = sm1 + 1;
-    constexpr int half = 1 << (s - 1);
-    static_for<32>([&](auto b) {
-      constexpr int j = b % half;
-      constexpr int i0 = (b / half) * 2 * half + j;
-      constexpr int e0 = (39 * j * (64 >> s)) % 192;
-      constexpr int e = DIR ? (192 - e0) % 192 : e0;
-      // 2^e = -2^(e - 96) for e >= 96: the sign swaps the butterfly's add and sub.
-      constexpr int em = e % 96;
-      const uint64_t u = y[i0];
-      uint64_t t;
-      if constexpr (em == 0) {
-        t = s == 1 ? y[i0 + half] : gl_canon(y[i0 + half]);
-      } else {
-        t = gl_shl_lz<em>(y[i0 + half]);
-      }
-      if constexpr (e >= 96) {
-        y[i0] = gl_sub_lz(u, t);
-        y[i0 + half] = gl_add_lz(u, t);
-      } else {
-        y[i0] = gl_add_lz(u, t);
-        y[i0 + half] = gl_sub_lz(u, t);
-      }
+  static_for<8>([&](auto c) {
+    constexpr int g = bitrev3(c);
+    if constexpr (CANON_IN) {
+      // y[8g + t] = x[8 bitrev3(t) + c]: the upper half of the class is t odd.
+      static_for<4>([&](auto h) { y[8 * g + 2 * h + 1] = gl_canon(y[8 * g + 2 * h + 1]); });
+    }
+    static_for<3>([&](auto sm1) {
+      static_for<4>([&](auto q) { dit_bfly<DIR, sm1 + 1, 4 * g + q>(y); });
     });
+  });
+  static_for<3>([&](auto sm4) {
+    static_for<32>([&](auto b) { dit_bfly<DIR, sm4 + 4, b>(y); });
   });
   static_for<64>([&](auto i) { x[i] = y[i]; });
 }
 
-// x[32..63] made canonical: the precondition of dft64_regs.
-__host__ __device__ __forceinline__ void canon_upper(uint64_t (&x)[64]) {
-  static_for<32>([&](auto i) { x[32 + i] = gl_canon(x[32 + i]); });
+// Calls f(m) for m = 0..63 in residue-class order (c + 8u, c outer), the
+// order in which dft64_regs consumes its inputs.
+template <typename Fn>
+__host__ __device__ __forceinline__ void load_order(Fn &&f) {
+  static_for<8>([&](auto c) { static_for<8>([&](auto u) { f(std::integral_constant<int, c + 8 * u>{}); }); });
 }
 
 // base[off] for a wave-uniform base and a per-lane element offset < 2^29:
 // the address is an SGPR pair plus one 32-bit VGPR, so unrolled loops over
-// a compile-time row index keep no per-row 64-bit addresses live.
+// a compile-time row index keep no per-row 64-bit addresses live.  The base
+// may come from a job table in memory, so it is cast to the global address
+// space explicitly: generic (flat) accesses would also count against the LDS
+// counter and force a full wait before the first use of any load.
+typedef __attribute__((address_space(1))) uint64_t gmem_u64;
+
 __device__ __forceinline__ uint64_t ld_u64(const uint64_t *base, uint32_t off) {
-  return *(const uint64_t *)((const char *)base + (off << 3));
+  return *(const gmem_u64 *)((const char *)base + (off << 3));
 }
 __device__ __forceinline__ void st_u64(uint64_t *base, uint32_t off, uint64_t v) {
-  *(uint64_t *)((char *)base + (off << 3)) = v;
+  *(gmem_u64 *)((char *)base + (off << 3)) = v;
 }
 
 // x[k] *= w_4096^(a k) for k = 1..63, tw = table + a.  Loaded eight at a
@@ -172,7 +201,7 @@ struct Ntt4096Args {
   const uint64_t *tw4096;  // [k][a] = w_4096^(a k), a, k < 64 (direction-specific)
   const uint64_t *tw_lo;   // w_n^i, i < 4096 (pass 1 only)
   const uint64_t *tw_hi;   // w_n^(4096 i)
-  uint64_t scale;          // pass 2: n^-1 for the inverse, else 1
+  uint64_t scale;          // pass 2 of the inverse: n^-1 (unused otherwise)
   uint32_t tiles;          // number of T-wide tiles
   // Batched form: when set, job blockIdx.y reads in_tab[y] and writes out_tab[y].
   const uint64_t *const *in_tab;
@@ -244,14 +273,14 @@ k_ntt4096_cols(Ntt4096Args args) {
   const uint64_t n2 = args.n2;
   const uint32_t lane = a * (uint32_t)n2 + s;  // element (a, s) of the tile's first row block
   uint64_t x[64];
-#pragma unroll
-  for (int m = 0; m < 64; m++) x[m] = ld_u64(args.in + (uint64_t)(64 * m) * n2 + (uint64_t)tile * T, lane);
-  canon_upper(x);  // inputs >= p count as their residue
-  dft64_regs<DIR>(x);
+  // All 64 loads are issued before any arithmetic (memory-level parallelism
+  // for the strided row segments); the waits then come class by class.
+  load_order([&](auto m) { x[m] = ld_u64(args.in + (uint64_t)(64 * m) * n2 + (uint64_t)tile * T, lane); });
+  __builtin_amdgcn_sched_barrier(0);
+  dft64_regs<DIR, true>(x);  // inputs >= p count as their residue
   inner_twiddle(x, args.tw4096 + a);
   exchange<T, false>(x, lds, s, a, s, a);
-  canon_upper(x);
-  dft64_regs<DIR>(x);
+  dft64_regs<DIR, true>(x);
   // x[k''] = output k1 = a + 64 k'' (a now plays k').  4-step twiddle
   // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.  The scratch matrix
   // holds lazy (not necessarily canonical) values; pass 2 canonicalizes the
@@ -284,22 +313,25 @@ k_ntt4096_rows(Ntt4096Args args) {
   const uint64_t n2 = args.n2;
   const uint32_t lane_in = r * (uint32_t)n2 + a;
   uint64_t x[64];
-#pragma unroll
-  for (int m = 0; m < 64; m++) x[m] = ld_u64(args.in + row0 * n2 + 64 * m, lane_in);
-  canon_upper(x);
-  dft64_regs<DIR>(x);
+  load_order([&](auto m) { x[m] = ld_u64(args.in + row0 * n2 + 64 * m, lane_in); });
+  __builtin_amdgcn_sched_barrier(0);
+  dft64_regs<DIR, true>(x);
   inner_twiddle(x, args.tw4096 + a);
   // after the exchange thread (r2, kp) with r2 fastest, for T-row store runs
   const uint32_t r2 = tid % T, kp = tid / T;
   exchange<T, true>(x, lds, r, a, r2, kp);
-  canon_upper(x);
-  dft64_regs<DIR>(x);
+  dft64_regs<DIR, true>(x);
+  // Keep the stores out of the last butterfly stages: interleaving them there
+  // raised the register pressure past 256 VGPRs (spills whose reloads would
+  // also wait for every earlier store).
+  __builtin_amdgcn_sched_barrier(0);
   const uint64_t n1 = args.n1;
   const uint32_t lane_out = kp * (uint32_t)n1 + r2;
 #pragma unroll
   for (int k = 0; k < 64; k++) {
     uint64_t v = x[k];
-    v = gl_canon(args.scale != 1 ? gl_mul_lz(v, args.scale) : v);
+    if constexpr (DIR == 1) v = gl_mul_lz(v, args.scale);  // n^-1 (the launcher sets it for DIR 1 only)
+    v = gl_canon(v);
     st_u64(args.out + (uint64_t)(64 * k) * n1 + row0, lane_out, v);
   }
 }

@@ -5,7 +5,8 @@
 //   - gl_shl_lz<E> for every E in [1, 96), result canonical and equal to v 2^E;
 //   - gl_reduce128_lz / gl_mul_lz for any 64-bit operands;
 //   - dft64_regs<0/1> against the defining sum with w_64 = 2^39 (and 2^153),
-//     with lazy (>= p) values wherever the precondition allows them.
+//     with lazy (>= p) values wherever the precondition allows them, and
+//     anywhere when the inputs are canonicalized inside (CANON_IN).
 // Prints "ok" and exits 0 when every case passes.
 #include <cstdio>
 #include <random>
@@ -87,10 +88,14 @@ int main() {
         x[i] = v;
         ref[i] = v % GL_P;
       }
+      // odd trials: any u64 in every slot, canonicalized inside (CANON_IN)
+      if (trial & 1)
+        for (int i = 32; i < 64; i++)
+          if ((trial + i) % 3 == 0) x[i] = GL_P + (rng() % GL_EPS), ref[i] = x[i] % GL_P;
       if (dir)
-        dft64_regs<1>(x);
+        (trial & 1) ? dft64_regs<1, true>(x) : dft64_regs<1>(x);
       else
-        dft64_regs<0>(x);
+        (trial & 1) ? dft64_regs<0, true>(x) : dft64_regs<0>(x);
       for (int k = 0; k < 64; k++) {
         uint64_t acc = 0;
         for (int m = 0; m < 64; m++) acc = (uint64_t)(((u128)acc + mulmod(ref[m], powmod(w, (uint64_t)m * k))) % GL_P);
