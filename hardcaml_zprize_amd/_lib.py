@@ -23,6 +23,8 @@ SIGNATURES = {
     "zp_ntt_size": (_u64, [_vp]),
     "zp_ntt_max_buffers_in_flight": (_u32, [_vp]),
     "zp_ntt_inplace_device": (_i32, [_vp, _vp, _i32, _vp]),
+    "zp_ntt_time_passes": (_i32, [_vp, _vp, _i32, _i32, _vp, ctypes.POINTER(ctypes.c_float),
+                                  ctypes.POINTER(ctypes.c_float)]),
     "zp_ntt_simple_evaluate": (_i32, [_vp, _vp, _vp, _u64, _i32]),
     "zp_ntt_request_buffer": (_i32, [_vp, ctypes.POINTER(_i32)]),
     "zp_ntt_buffer_input": (_vp, [_vp, _i32]),

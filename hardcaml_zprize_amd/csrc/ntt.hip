@@ -183,24 +183,30 @@ static int launch_4096_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out,
   return ZP_OK;
 }
 
-static int launch_4096_pair(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, hipStream_t stream,
-                            const JobTabs &jt) {
+static void tile_widths() {
   if (g_tile_cols < 0) g_tile_cols = env_tile("ZP_NTT_TILE_COLS", 8);
   if (g_tile_rows < 0) g_tile_rows = env_tile("ZP_NTT_TILE_ROWS", 8);
-  ZP_TRY(g_tile_cols == 4 ? launch_4096_cols<4>(t, d_data, d_scratch, dir, stream, jt)
-                          : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt));
-  return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
-                          : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
 }
 
-// Enqueue one in-place transform.  scratch must hold n elements when log_n > 12.
-static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir,
-                            hipStream_t stream, const JobTabs &jt = JobTabs()) {
+// One pass of an in-place transform (pass 1 = column DFTs + 4-step twiddle,
+// data -> scratch; pass 2 = row DFTs, scratch -> data transposed).  A length
+// that is a single DFT (log_n <= 12) has only pass 1, which works in place.
+// scratch must hold n elements when log_n > 12.
+static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, int pass,
+                       hipStream_t stream, const JobTabs &jt = JobTabs()) {
   const NttShape sh = shape_of(t->log_n);
-  if (sh.log_n1 == 12 && sh.log_n2 == 12) return launch_4096_pair(t, d_data, d_scratch, dir, stream, jt);
+  if (sh.log_n1 == 12 && sh.log_n2 == 12) {
+    tile_widths();
+    if (pass == 1)
+      return g_tile_cols == 4 ? launch_4096_cols<4>(t, d_data, d_scratch, dir, stream, jt)
+                              : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt);
+    return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
+                            : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
+  }
   const uint64_t n = 1ull << sh.log_n;
   const uint64_t scale = dir ? t->n_inv : 1;
   if (sh.log_n2 == 0) {
+    if (pass != 1) return ZP_OK;
     DftArgs a{};
     a.in = d_data;
     a.out = d_data;
@@ -217,24 +223,26 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
     return launch_dft(a, 1, stream, jt.jobs);
   }
   const uint64_t n1 = 1ull << sh.log_n1, n2 = 1ull << sh.log_n2;
-  // Pass 1: columns (length n1, n2 of them) + twiddle, data -> scratch.
-  DftArgs p1{};
-  p1.in = d_data;
-  p1.out = d_scratch;
-  p1.log_L = sh.log_n1;
-  p1.log_T = dft_log_T(sh.log_n1, sh.log_n2);
-  p1.in_elem_stride = n2;
-  p1.in_seq_stride = 1;
-  p1.out_elem_stride = n2;
-  p1.out_seq_stride = 1;
-  p1.tw_sub = t->d_sub[dir];
-  p1.tw_lo = t->d_lo[dir];
-  p1.tw_hi = t->d_hi[dir];
-  p1.scale = 1;
-  p1.in_tab = jt.data;
-  p1.out_tab = jt.scratch;
-  ZP_TRY(launch_dft(p1, n2 >> p1.log_T, stream, jt.jobs));
-  // Pass 2: rows (length n2, n1 of them), scratch -> data transposed.
+  if (pass == 1) {
+    // Columns (length n1, n2 of them) + twiddle, data -> scratch.
+    DftArgs p1{};
+    p1.in = d_data;
+    p1.out = d_scratch;
+    p1.log_L = sh.log_n1;
+    p1.log_T = dft_log_T(sh.log_n1, sh.log_n2);
+    p1.in_elem_stride = n2;
+    p1.in_seq_stride = 1;
+    p1.out_elem_stride = n2;
+    p1.out_seq_stride = 1;
+    p1.tw_sub = t->d_sub[dir];
+    p1.tw_lo = t->d_lo[dir];
+    p1.tw_hi = t->d_hi[dir];
+    p1.scale = 1;
+    p1.in_tab = jt.data;
+    p1.out_tab = jt.scratch;
+    return launch_dft(p1, n2 >> p1.log_T, stream, jt.jobs);
+  }
+  // Rows (length n2, n1 of them), scratch -> data transposed.
   DftArgs p2{};
   p2.in = d_scratch;
   p2.out = d_data;
@@ -249,6 +257,13 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
   p2.in_tab = jt.scratch;
   p2.out_tab = jt.data;
   return launch_dft(p2, n1 >> p2.log_T, stream, jt.jobs);
+}
+
+// Enqueue one in-place transform.
+static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir,
+                            hipStream_t stream, const JobTabs &jt = JobTabs()) {
+  ZP_TRY(launch_pass(t, d_data, d_scratch, dir, 1, stream, jt));
+  return launch_pass(t, d_data, d_scratch, dir, 2, stream, jt);
 }
 
 }  // namespace zp
@@ -333,6 +348,42 @@ extern "C" int zp_ntt_inplace_device(zp_ntt_plan *p, uint64_t *d_data, int dir, 
   ZP_CHECK(p != nullptr && d_data != nullptr, ZP_ERR_INVALID_ARG, "zp_ntt_inplace_device: NULL argument");
   ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_inplace_device: dir %d not 0/1", dir);
   return launch_transform(p->tables, d_data, p->d_scratch, dir, (hipStream_t)stream);
+}
+
+extern "C" int zp_ntt_time_passes(zp_ntt_plan *p, uint64_t *d_data, int dir, int reps, void *stream, float *ms_pass1,
+                                  float *ms_pass2) {
+  ZP_CHECK(p != nullptr && d_data != nullptr && ms_pass1 != nullptr && ms_pass2 != nullptr, ZP_ERR_INVALID_ARG,
+           "zp_ntt_time_passes: NULL argument");
+  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_time_passes: dir %d not 0/1", dir);
+  ZP_CHECK(reps > 0, ZP_ERR_INVALID_ARG, "zp_ntt_time_passes: reps %d", reps);
+  const hipStream_t s = (hipStream_t)stream;
+  std::vector<hipEvent_t> ev(3 * (size_t)reps);
+  for (auto &e : ev) ZP_HIP(hipEventCreate(&e));
+  int rc = ZP_OK;
+  for (int r = 0; r < reps && rc == ZP_OK; r++) {
+    (void)hipEventRecord(ev[3 * r], s);
+    rc = launch_pass(p->tables, d_data, p->d_scratch, dir, 1, s);
+    (void)hipEventRecord(ev[3 * r + 1], s);
+    if (rc == ZP_OK) rc = launch_pass(p->tables, d_data, p->d_scratch, dir, 2, s);
+    (void)hipEventRecord(ev[3 * r + 2], s);
+  }
+  double t1 = 0, t2 = 0;
+  if (rc == ZP_OK && hipEventSynchronize(ev.back()) == hipSuccess) {
+    for (int r = 0; r < reps; r++) {
+      float a = 0, b = 0;
+      (void)hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]);
+      t1 += a;
+      t2 += b;
+    }
+  } else if (rc == ZP_OK) {
+    set_last_error("zp_ntt_time_passes: event synchronize failed");
+    rc = ZP_ERR_HIP;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  *ms_pass1 = (float)(t1 / reps);
+  *ms_pass2 = (float)(t2 / reps);
+  return rc;
 }
 
 extern "C" int zp_ntt_simple_evaluate(zp_ntt_plan *p, uint64_t *out, const uint64_t *in, uint64_t len, int dir) {

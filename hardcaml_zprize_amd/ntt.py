@@ -110,6 +110,15 @@ class NttGpuDriver:
         call("zp_ntt_inplace_device", self.plan, ctypes.c_void_p(d_ptr), direction, ctypes.c_void_p(stream or 0))
 
 
+    def time_passes(self, d_ptr, direction=FORWARD, reps=10, stream=None):
+        """Mean milliseconds of the transform's two kernel launches (pass 1,
+        pass 2), timed with HIP events on `stream` (zp_ntt_time_passes)."""
+        m1, m2 = ctypes.c_float(), ctypes.c_float()
+        call("zp_ntt_time_passes", self.plan, ctypes.c_void_p(d_ptr), direction, int(reps),
+             ctypes.c_void_p(stream or 0), ctypes.byref(m1), ctypes.byref(m2))
+        return m1.value, m2.value
+
+
 def ntt_batch_device(ptrs, log_ns, dirs, stream=None):
     """Many in-place transforms of mixed sizes/directions, one launch sequence."""
     n = len(ptrs)

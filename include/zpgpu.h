@@ -69,6 +69,14 @@ unsigned zp_ntt_max_buffers_in_flight(const zp_ntt_plan *plan);
  * zprize/ntt/host/driver.cpp:201-231).  Asynchronous. */
 int zp_ntt_inplace_device(zp_ntt_plan *plan, uint64_t *d_data, int dir, void *stream);
 
+/* Measurement (no counterpart in the reference): runs the same in-place
+ * transform `reps` times on `stream` with HIP events around each of its two
+ * kernel launches (pass 1: column DFTs + 4-step twiddle; pass 2: row DFTs)
+ * and returns the mean milliseconds of each.  Blocks until done.  For
+ * log_n <= 12 (a single launch) ms_pass2 is 0. */
+int zp_ntt_time_passes(zp_ntt_plan *plan, uint64_t *d_data, int dir, int reps, void *stream, float *ms_pass1,
+                       float *ms_pass2);
+
 /* NttFpgaDriver::simple_evaluate (driver.h:188-197, driver.cpp:332-344):
  * host in, host out, blocking; input shorter than n is zero padded;
  * data_length > n is ZP_ERR_TOO_LARGE (driver.cpp:289-297). */
