@@ -42,6 +42,7 @@ SIGNATURES = {
     "zp_msm_init": (_i32, [ctypes.c_char_p, _sz, _vp, _sz, ctypes.POINTER(_vp)]),
     "zp_msm_init_device": (_i32, [ctypes.c_char_p, _sz, _vp, _sz, ctypes.POINTER(_vp)]),
     "zp_msm_mult": (_i32, [_vp, _vp, _u64, _vp]),
+    "zp_fr_to_montgomery_device": (_i32, [_vp, _u64, _vp]),
     "zp_msm_mult_device": (_i32, [_vp, _vp, _u64, _vp, _vp]),
     "zp_msm_mult_prefix_device": (_i32, [_vp, _sz, _vp, _vp, _vp]),
     "zp_msm_num_points": (_sz, [_vp]),

@@ -258,6 +258,7 @@ struct FConsts {
   F one;        // R' mod q
   F two;        // 2 R'
   F to_int;     // 2^400 mod q: f_mul(x R, to_int) = x R'
+  F r2;         // R'^2 = 2^784 mod q: f_mul(x, r2) = x R' for a plain (canonical) x
   F to_wire;    // 2^384 mod q as a plain value: f_mul(x R', to_wire) = x R
   F k2d;        // 2d (twisted-Edwards k of hwcd-3)
   F d2;         // 2d again, used for the precomputed t coordinate

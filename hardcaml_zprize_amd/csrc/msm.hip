@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "msm_host.hpp"
 #include "msm_kernels.hpp"
+#include "fr253.hpp"
 
 namespace zp {
 
@@ -85,6 +86,10 @@ static int auto_window(size_t n) {
 struct MsmOptions {
   int window = 0;  // 0 = auto
   bool reference = false;
+  // msm_init only: point coordinates are plain integers < q ("coords=canonical",
+  // the form ark-serialize writes) instead of the wire's Montgomery x R.
+  bool canonical_coords = false;
+  bool bad_coords = false;  // coords= neither "canonical" nor "montgomery"
   int log_u = -1;  // entries per work unit, -1 = auto
   int log_g = -1;  // buckets per window group, -1 = auto
 };
@@ -105,6 +110,9 @@ static MsmOptions parse_options(const char *s, size_t len) {
           o.reference = true;
         else
           o.window = atoi(v.c_str());
+      } else if (k == "coords") {
+        o.canonical_coords = v == "canonical";
+        o.bad_coords = v != "canonical" && v != "montgomery";
       } else if (k == "log_u") {
         o.log_u = atoi(v.c_str());
       } else if (k == "log_g") {
@@ -120,6 +128,7 @@ static int validate_options(const MsmOptions &o, const char *who) {
   ZP_CHECK(o.window == 0 || (o.window >= 8 && o.window <= 24), ZP_ERR_INVALID_ARG, "%s: window=%d not in [8, 24]",
            who, o.window);
   ZP_CHECK(o.log_u <= 12 && o.log_g <= 10, ZP_ERR_INVALID_ARG, "%s: log_u/log_g out of range", who);
+  ZP_CHECK(!o.bad_coords, ZP_ERR_INVALID_ARG, "%s: coords= must be canonical or montgomery", who);
   return ZP_OK;
 }
 
@@ -130,7 +139,9 @@ static HostLayout choose_layout(const MsmOptions &o, size_t n) {
 }
 
 // ----------------------------------------------------------- kernels --
-__global__ void k_pre_a(const g1_affine_t *in, uint64_t n, FConsts K, F *u_out, F *v_out, F *den_out,
+// to_r converts an input coordinate into the R' domain: K.to_int for the
+// wire form x R, K.r2 for a plain x.
+__global__ void k_pre_a(const g1_affine_t *in, uint64_t n, FConsts K, F to_r, F *u_out, F *v_out, F *den_out,
                         uint8_t *flag) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const g1_affine_t &p = in[i];
@@ -147,8 +158,8 @@ __global__ void k_pre_a(const g1_affine_t *in, uint64_t n, FConsts K, F *u_out, 
       px.w[j] = wx[j];
       py.w[j] = wy[j];
     }
-    const F x = f_mul(f_unpack(px), K.to_int);
-    const F y = f_mul(f_unpack(py), K.to_int);
+    const F x = f_mul(f_unpack(px), to_r);
+    const F y = f_mul(f_unpack(py), to_r);
     // affineWeierstrassToMontgomery (bls12_377_g1.h:361-365): u = s (x - alpha), v = s y.
     const F u = f_mul(f_sub_kq(x, K.alpha, K.q2), K.mont_s);
     const F v = f_mul(K.mont_s, y);
@@ -720,9 +731,24 @@ static int dump_digits(const MsmOptions &o, const uint64_t *d_scalars, size_t n,
 
 using namespace zp;
 
+// -------------------------------------------- harness data-file scalars --
+__global__ void k_fr_to_montgomery(biginteger256_t *s, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    Fr8 v;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(s[i].data);
+#pragma unroll
+    for (int j = 0; j < 8; j++) v.w[j] = w[j];
+    const Fr8 m = fr_to_montgomery(v);
+    uint32_t *o = reinterpret_cast<uint32_t *>(s[i].data);
+#pragma unroll
+    for (int j = 0; j < 8; j++) o[j] = m.w[j];
+  }
+}
+
 // ---------------------------------------------------------------- init --
 static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_device) {
   const FConsts &K = consts();
+  const F to_r = ctx->opts.canonical_coords ? K.r2 : K.to_int;
   const size_t n = ctx->n;
   ZP_HIP(hipMalloc(&ctx->d_pts, std::max<size_t>(n, 1) * sizeof(PackedPre)));
   ZP_HIP(hipMalloc(&ctx->d_conv, std::max<size_t>(n, 1)));
@@ -752,8 +778,8 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
       }
       src = d_stage;
     }
-    hipLaunchKernelGGL(k_pre_a, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, src, (uint64_t)m, K, d_u, d_v, d_den,
-                       d_flag);
+    hipLaunchKernelGGL(k_pre_a, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, src, (uint64_t)m, K, to_r, d_u, d_v,
+                       d_den, d_flag);
     const uint32_t chunk = 32;
     hipLaunchKernelGGL(k_batch_inv, dim3(grid_for((m + chunk - 1) / chunk, 64)), dim3(64), 0, s, d_den, (uint64_t)m,
                        chunk, d_tmp, K);
@@ -790,8 +816,8 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
     U384 x, y;
     memcpy(x.w, p.x.data, 48);
     memcpy(y.w, p.y.data, 48);
-    nc.x = f_canonical(f_mul(f_from_u384(x), K.to_int));
-    nc.y = f_canonical(f_mul(f_from_u384(y), K.to_int));
+    nc.x = f_canonical(f_mul(f_from_u384(x), to_r));
+    nc.y = f_canonical(f_mul(f_from_u384(y), to_r));
     ctx->nonconv.push_back(nc);
   }
   return ZP_OK;
@@ -903,6 +929,14 @@ extern "C" int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_proje
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
   HostJob job{ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), nullptr, out};
   return run_batch(ctx->ws, &job, 1, MsmOptions(), s);
+}
+
+extern "C" int zp_fr_to_montgomery_device(biginteger256_t *d_scalars, uint64_t n, void *stream) {
+  ZP_CHECK(d_scalars != nullptr || n == 0, ZP_ERR_INVALID_ARG, "zp_fr_to_montgomery_device: NULL scalars");
+  if (n == 0) return ZP_OK;
+  hipLaunchKernelGGL(k_fr_to_montgomery, dim3(grid_for(n)), dim3(MSM_THREADS), 0, (hipStream_t)stream, d_scalars, n);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
 }
 
 // ------------------------------------------------------------- batched --

@@ -98,6 +98,7 @@ inline FConsts make_fconsts() {
   K.one = f_canonical(f_from_u384(pow2_mod_q(392)));
   K.two = f_canonical(f_add(K.one, K.one));
   K.to_int = f_from_u384(pow2_mod_q(400));
+  K.r2 = r2;
   K.to_wire = f_from_u384(pow2_mod_q(384));
   const F d = mont(u384_from_hex(HEX_D));
   K.d2 = f_canonical(f_add(d, d));

@@ -127,7 +127,9 @@ typedef struct zp_msm_context zp_msm_context;
  *                             g1_affine_t *rust_points, ssize_t npoints)
  * (zprize/msm_pippenger/host/driver/driver.cpp:603-617).  The xclbin string
  * (not NUL terminated) is an optional option string here, e.g. "window=16";
- * it may be NULL/empty.  Points are converted on the GPU to the
+ * it may be NULL/empty.  "coords=canonical" takes x and y as plain integers
+ * < q (the form ark-serialize files hold) instead of x R, y R; the default
+ * is "coords=montgomery".  Points are converted on the GPU to the
  * twisted-Edwards precomputed form and stay resident in HBM.  Returns NULL
  * on failure (zp_last_error() says why). */
 zp_msm_context *msm_init(const char *xclbin, ssize_t xclbin_len, g1_affine_t *rust_points, ssize_t npoints);
@@ -148,6 +150,13 @@ int zp_msm_init(const char *options, size_t options_len, const g1_affine_t *poin
 int zp_msm_init_device(const char *options, size_t options_len, const g1_affine_t *d_points, size_t npoints,
                        zp_msm_context **out);
 int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches, const biginteger256_t *scalars);
+
+/* In place, s -> s * 2^256 mod r for n scalars < r in HBM: the integers the
+ * ZPrize harness multiplies by when it loads Fr values from its data files
+ * (ark-ff 0.3 keeps them as s R_r and the harness transmutes them to
+ * BigInteger256, zprize/msm_pippenger/test_fpga_harness/src/util.rs:115-118,
+ * 150-173).  Asynchronous on `stream`. */
+int zp_fr_to_montgomery_device(biginteger256_t *d_scalars, uint64_t n, void *stream);
 /* Scalars already resident in HBM (num_batches * npoints * 32 bytes). */
 int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_batches,
                        const biginteger256_t *d_scalars, void *stream);
