@@ -19,6 +19,7 @@ SIGNATURES = {
     "zp_last_error": (ctypes.c_char_p, []),
     "zp_version": (ctypes.c_char_p, []),
     "zp_ntt_create": (_i32, [_u32, _u32, ctypes.POINTER(_vp)]),
+    "zp_ntt_create_reverse": (_i32, [_u32, _u32, ctypes.POINTER(_vp)]),
     "zp_ntt_destroy": (_i32, [_vp]),
     "zp_ntt_size": (_u64, [_vp]),
     "zp_ntt_max_buffers_in_flight": (_u32, [_vp]),

@@ -183,6 +183,63 @@ static int launch_4096_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out,
   return ZP_OK;
 }
 
+// ----------------------------------------------------------- REVERSE core --
+// The FPGA NTT build's memory-path debug core (zprize/ntt/host/driver.h:18,
+// reference.cpp:60-84): the two phases of the 4-step transform with each row
+// DFT replaced by reversing the row.  With R = 2^log_r and w = OMEGA[2 log_r]:
+//   phase 1 (reference.cpp:69-77)  E[i][j] = in[R-1-i][j] * w^(i j)
+//   phase 2 (reference.cpp:79-83)  out[i][j] = E[j][R-1-i]
+// (reverse columns + twiddle, then reverse rows + transpose).
+__global__ void __launch_bounds__(256) k_reverse_phase1(const uint64_t *in, uint64_t *out, unsigned log_r,
+                                                        const uint64_t *tw_lo, const uint64_t *tw_hi) {
+  const uint64_t r = 1ull << log_r, n = r << log_r;
+  for (uint64_t idx = blockIdx.x * 256ull + threadIdx.x; idx < n; idx += gridDim.x * 256ull) {
+    const uint64_t i = idx >> log_r, j = idx & (r - 1);
+    const uint64_t e = (i * j) & (n - 1);  // w^n = 1
+    const uint64_t w = gl_mul(tw_hi[e >> 12], tw_lo[e & 4095]);
+    out[idx] = gl_mul(in[((r - 1 - i) << log_r) + j], w);
+  }
+}
+
+// 32 x 32 tiles through LDS so that both the reads (rows of E) and the
+// writes (rows of out) are contiguous; R < 32 takes the direct gather.
+__global__ void __launch_bounds__(256) k_reverse_phase2(const uint64_t *in, uint64_t *out, unsigned log_r) {
+  const uint64_t r = 1ull << log_r;
+  if (r < 32) {
+    const uint64_t idx = threadIdx.x;  // one block of >= r * r threads
+    if (idx < r * r) {
+      const uint64_t i = idx >> log_r, j = idx & (r - 1);
+      out[idx] = in[(j << log_r) + (r - 1 - i)];
+    }
+    return;
+  }
+  __shared__ uint64_t tile[32][33];
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const uint64_t c0 = (uint64_t)blockIdx.x * 32, j0 = (uint64_t)blockIdx.y * 32;
+  for (int k = 0; k < 32; k += 8) tile[ty + k][tx] = in[((j0 + ty + k) << log_r) + c0 + tx];  // E[j][c]
+  __syncthreads();
+  // out[R-1-c][j] = E[j][c]
+  for (int k = 0; k < 32; k += 8) out[((r - 1 - (c0 + ty + k)) << log_r) + j0 + tx] = tile[tx][ty + k];
+}
+
+static int launch_reverse_phase(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int pass,
+                                hipStream_t stream) {
+  const unsigned log_r = t->log_n / 2;
+  const uint64_t r = 1ull << log_r, n = r * r;
+  if (pass == 1) {
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(k_reverse_phase1, dim3(blocks), dim3(256), 0, stream, d_data, d_scratch, log_r,
+                       t->d_lo[0], t->d_hi[0]);
+  } else if (r < 32) {
+    hipLaunchKernelGGL(k_reverse_phase2, dim3(1), dim3(256), 0, stream, d_scratch, d_data, log_r);
+  } else {
+    hipLaunchKernelGGL(k_reverse_phase2, dim3((unsigned)(r / 32), (unsigned)(r / 32)), dim3(256), 0, stream,
+                       d_scratch, d_data, log_r);
+  }
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
 static void tile_widths() {
   if (g_tile_cols < 0) g_tile_cols = env_tile("ZP_NTT_TILE_COLS", 8);
   if (g_tile_rows < 0) g_tile_rows = env_tile("ZP_NTT_TILE_ROWS", 8);
@@ -283,6 +340,7 @@ struct UserBuffer {
 struct zp_ntt_plan {
   unsigned log_n = 0;
   uint64_t n = 0;
+  bool reverse = false;  // the REVERSE debug core instead of the transform
   NttTables *tables = nullptr;
   uint64_t *d_scratch = nullptr;  // n elements (4-step intermediate)
   uint64_t *d_work = nullptr;     // n elements for simple_evaluate
@@ -290,37 +348,66 @@ struct zp_ntt_plan {
   std::vector<UserBuffer> buffers;
 };
 
+// One pass (1 or 2) of the plan's operation on d_data (in place, scratch in between).
+static int plan_pass(const zp_ntt_plan *p, uint64_t *d_data, int dir, int pass, hipStream_t stream) {
+  if (p->reverse) return launch_reverse_phase(p->tables, d_data, p->d_scratch, pass, stream);
+  return launch_pass(p->tables, d_data, p->d_scratch, dir, pass, stream);
+}
+
+static int plan_launch(const zp_ntt_plan *p, uint64_t *d_data, int dir, hipStream_t stream) {
+  ZP_TRY(plan_pass(p, d_data, dir, 1, stream));
+  return plan_pass(p, d_data, dir, 2, stream);
+}
+
+static int check_dir(const zp_ntt_plan *p, int dir, const char *who) {
+  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "%s: dir %d not 0/1", who, dir);
+  ZP_CHECK(!(p->reverse && dir == 1), ZP_ERR_INVALID_ARG, "%s: the REVERSE core has no inverse", who);
+  return ZP_OK;
+}
+
 extern "C" const char *zp_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char *zp_version(void) { return "zpgpu gfx950"; }
 
-extern "C" int zp_ntt_create(unsigned log_n, unsigned num_buffers, zp_ntt_plan **out) {
-  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "zp_ntt_create: out is NULL");
+static int create_plan(unsigned log_n, unsigned num_buffers, bool reverse, zp_ntt_plan **out, const char *who) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "%s: out is NULL", who);
   *out = nullptr;
-  ZP_CHECK(log_n >= 1 && log_n <= 24, ZP_ERR_INVALID_ARG, "zp_ntt_create: log_n %u not in [1, 24]", log_n);
   if (num_buffers == 0) num_buffers = 8;
-  ZP_CHECK(num_buffers <= 64, ZP_ERR_INVALID_ARG, "zp_ntt_create: num_buffers %u > 64", num_buffers);
+  ZP_CHECK(num_buffers <= 64, ZP_ERR_INVALID_ARG, "%s: num_buffers %u > 64", who, num_buffers);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    set_last_error("zp_ntt_create: no HIP device");
+    set_last_error("%s: no HIP device", who);
     return ZP_ERR_NO_DEVICE;
   }
   zp_ntt_plan *p = new zp_ntt_plan();
   p->log_n = log_n;
   p->n = 1ull << log_n;
+  p->reverse = reverse;
   int st = make_tables(log_n, &p->tables);
-  if (st == ZP_OK && log_n > 12 && hipMalloc(&p->d_scratch, p->n * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
+  if (st == ZP_OK && (log_n > 12 || reverse) && hipMalloc(&p->d_scratch, p->n * 8) != hipSuccess)
+    st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipMalloc(&p->d_work, p->n * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) st = ZP_ERR_HIP;
   if (st == ZP_OK) {
     p->buffers.resize(num_buffers);
   }
   if (st != ZP_OK) {
-    if (st == ZP_ERR_OUT_OF_MEMORY) set_last_error("zp_ntt_create: device allocation failed");
+    if (st == ZP_ERR_OUT_OF_MEMORY) set_last_error("%s: device allocation failed", who);
     zp_ntt_destroy(p);
     return st;
   }
   *out = p;
   return ZP_OK;
+}
+
+extern "C" int zp_ntt_create(unsigned log_n, unsigned num_buffers, zp_ntt_plan **out) {
+  ZP_CHECK(log_n >= 1 && log_n <= 24, ZP_ERR_INVALID_ARG, "zp_ntt_create: log_n %u not in [1, 24]", log_n);
+  return create_plan(log_n, num_buffers, false, out, "zp_ntt_create");
+}
+
+extern "C" int zp_ntt_create_reverse(unsigned log_row_size, unsigned num_buffers, zp_ntt_plan **out) {
+  ZP_CHECK(log_row_size >= 1 && log_row_size <= 12, ZP_ERR_INVALID_ARG,
+           "zp_ntt_create_reverse: log_row_size %u not in [1, 12]", log_row_size);
+  return create_plan(2 * log_row_size, num_buffers, true, out, "zp_ntt_create_reverse");
 }
 
 extern "C" int zp_ntt_destroy(zp_ntt_plan *p) {
@@ -346,15 +433,15 @@ extern "C" unsigned zp_ntt_max_buffers_in_flight(const zp_ntt_plan *p) {
 
 extern "C" int zp_ntt_inplace_device(zp_ntt_plan *p, uint64_t *d_data, int dir, void *stream) {
   ZP_CHECK(p != nullptr && d_data != nullptr, ZP_ERR_INVALID_ARG, "zp_ntt_inplace_device: NULL argument");
-  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_inplace_device: dir %d not 0/1", dir);
-  return launch_transform(p->tables, d_data, p->d_scratch, dir, (hipStream_t)stream);
+  ZP_TRY(check_dir(p, dir, "zp_ntt_inplace_device"));
+  return plan_launch(p, d_data, dir, (hipStream_t)stream);
 }
 
 extern "C" int zp_ntt_time_passes(zp_ntt_plan *p, uint64_t *d_data, int dir, int reps, void *stream, float *ms_pass1,
                                   float *ms_pass2) {
   ZP_CHECK(p != nullptr && d_data != nullptr && ms_pass1 != nullptr && ms_pass2 != nullptr, ZP_ERR_INVALID_ARG,
            "zp_ntt_time_passes: NULL argument");
-  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_time_passes: dir %d not 0/1", dir);
+  ZP_TRY(check_dir(p, dir, "zp_ntt_time_passes"));
   ZP_CHECK(reps > 0, ZP_ERR_INVALID_ARG, "zp_ntt_time_passes: reps %d", reps);
   const hipStream_t s = (hipStream_t)stream;
   std::vector<hipEvent_t> ev(3 * (size_t)reps);
@@ -362,9 +449,9 @@ extern "C" int zp_ntt_time_passes(zp_ntt_plan *p, uint64_t *d_data, int dir, int
   int rc = ZP_OK;
   for (int r = 0; r < reps && rc == ZP_OK; r++) {
     (void)hipEventRecord(ev[3 * r], s);
-    rc = launch_pass(p->tables, d_data, p->d_scratch, dir, 1, s);
+    rc = plan_pass(p, d_data, dir, 1, s);
     (void)hipEventRecord(ev[3 * r + 1], s);
-    if (rc == ZP_OK) rc = launch_pass(p->tables, d_data, p->d_scratch, dir, 2, s);
+    if (rc == ZP_OK) rc = plan_pass(p, d_data, dir, 2, s);
     (void)hipEventRecord(ev[3 * r + 2], s);
   }
   double t1 = 0, t2 = 0;
@@ -389,12 +476,12 @@ extern "C" int zp_ntt_time_passes(zp_ntt_plan *p, uint64_t *d_data, int dir, int
 extern "C" int zp_ntt_simple_evaluate(zp_ntt_plan *p, uint64_t *out, const uint64_t *in, uint64_t len, int dir) {
   ZP_CHECK(p != nullptr && out != nullptr && (in != nullptr || len == 0), ZP_ERR_INVALID_ARG,
            "zp_ntt_simple_evaluate: NULL argument");
-  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_simple_evaluate: dir %d not 0/1", dir);
+  ZP_TRY(check_dir(p, dir, "zp_ntt_simple_evaluate"));
   ZP_CHECK(len <= p->n, ZP_ERR_TOO_LARGE, "zp_ntt_simple_evaluate: data_length %llu > %llu",
            (unsigned long long)len, (unsigned long long)p->n);
   if (len) ZP_HIP(hipMemcpyAsync(p->d_work, in, len * 8, hipMemcpyHostToDevice, p->stream));
   if (len < p->n) ZP_HIP(hipMemsetAsync(p->d_work + len, 0, (p->n - len) * 8, p->stream));
-  ZP_TRY(launch_transform(p->tables, p->d_work, p->d_scratch, dir, p->stream));
+  ZP_TRY(plan_launch(p, p->d_work, dir, p->stream));
   ZP_HIP(hipMemcpyAsync(out, p->d_work, p->n * 8, hipMemcpyDeviceToHost, p->stream));
   ZP_HIP(hipStreamSynchronize(p->stream));
   return ZP_OK;
@@ -442,9 +529,9 @@ extern "C" int zp_ntt_enqueue(zp_ntt_plan *p, int id, int dir) {
   UserBuffer &b = p->buffers[id];
   ZP_CHECK(b.in_use, ZP_ERR_STATE, "zp_ntt_enqueue: buffer %d not requested", id);
   ZP_CHECK(!b.enqueued, ZP_ERR_STATE, "zp_ntt_enqueue: buffer %d already enqueued", id);
-  ZP_CHECK(dir == 0 || dir == 1, ZP_ERR_INVALID_ARG, "zp_ntt_enqueue: dir %d not 0/1", dir);
+  ZP_TRY(check_dir(p, dir, "zp_ntt_enqueue"));
   ZP_HIP(hipMemcpyAsync(b.d_buf, b.h_in, p->n * 8, hipMemcpyHostToDevice, p->stream));
-  ZP_TRY(launch_transform(p->tables, b.d_buf, p->d_scratch, dir, p->stream));
+  ZP_TRY(plan_launch(p, b.d_buf, dir, p->stream));
   ZP_HIP(hipMemcpyAsync(b.h_out, b.d_buf, p->n * 8, hipMemcpyDeviceToHost, p->stream));
   ZP_HIP(hipEventRecord(b.done, p->stream));
   b.enqueued = true;

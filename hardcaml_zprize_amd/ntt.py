@@ -40,13 +40,22 @@ def _host_view(addr, n):
 
 
 class NttGpuDriver:
-    """NttFpgaDriver(create_ntt_2_<log_n>) + load_xclbin in one constructor."""
+    """NttFpgaDriver(create_ntt_2_<log_n>) + load_xclbin in one constructor.
+    `NttGpuDriver.reverse(log_row_size)` is NttFpgaDriverArg::create_reverse:
+    the REVERSE debug core on 4^log_row_size elements."""
 
-    def __init__(self, log_n, num_buffers=8):
+    def __init__(self, log_n, num_buffers=8, _reverse=False):
         h = ctypes.c_void_p()
-        call("zp_ntt_create", log_n, num_buffers, ctypes.byref(h))
+        if _reverse:
+            call("zp_ntt_create_reverse", log_n // 2, num_buffers, ctypes.byref(h))
+        else:
+            call("zp_ntt_create", log_n, num_buffers, ctypes.byref(h))
         self.plan = h
         self.log_n = log_n
+
+    @classmethod
+    def reverse(cls, log_row_size, num_buffers=8):
+        return cls(2 * log_row_size, num_buffers, _reverse=True)
 
     def close(self):
         if self.plan:

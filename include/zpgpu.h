@@ -58,6 +58,12 @@ typedef struct zp_ntt_plan zp_ntt_plan;
  * log_n in [1, 26].  Allocates twiddle tables, one device scratch vector and
  * `num_buffers` (<= 64; 0 means 8, driver.cpp:133) in-flight user buffers. */
 int zp_ntt_create(unsigned log_n, unsigned num_buffers, zp_ntt_plan **out);
+/* NttFpgaDriverArg::create_reverse (driver.cpp:100-104): a plan of n = 4^log_row_size
+ * elements running the REVERSE debug core (driver.h:18) instead of the transform:
+ * the two 4-step passes with each row DFT replaced by reversing the row
+ * (zprize/ntt/host/reference.cpp:60-84).  Every plan entry point works on it;
+ * dir must be 0. */
+int zp_ntt_create_reverse(unsigned log_row_size, unsigned num_buffers, zp_ntt_plan **out);
 int zp_ntt_destroy(zp_ntt_plan *plan);
 /* NttFpgaDriver::input_vector_size (driver.h:127). */
 uint64_t zp_ntt_size(const zp_ntt_plan *plan);

@@ -18,12 +18,15 @@ namespace zpgpu {
 // NttFpgaDriverArg::create_ntt_2_24 / _2_18 / _2_12 (driver.h:36-56): the
 // memory layout and log_blocks arguments of the FPGA build have no GPU
 // meaning and are not taken; the transform size is.
+// create_reverse (driver.cpp:100-104) selects the REVERSE debug core.
 struct NttGpuDriverArg {
   unsigned log_n;
+  bool reverse = false;
   static NttGpuDriverArg create_ntt_2_24() { return {24}; }
   static NttGpuDriverArg create_ntt_2_18() { return {18}; }
   static NttGpuDriverArg create_ntt_2_12() { return {12}; }
   static NttGpuDriverArg create(unsigned log_n) { return {log_n}; }
+  static NttGpuDriverArg create_reverse(unsigned log_row_size) { return {2 * log_row_size, true}; }
   uint64_t num_elements() const { return 1ull << log_n; }
 };
 
@@ -43,7 +46,10 @@ class NttGpuDriver {
   };
 
   explicit NttGpuDriver(NttGpuDriverArg arg, unsigned num_buffers = 8) {
-    check(zp_ntt_create(arg.log_n, num_buffers, &plan_), "zp_ntt_create");
+    if (arg.reverse)
+      check(zp_ntt_create_reverse(arg.log_n / 2, num_buffers, &plan_), "NttGpuDriver");
+    else
+      check(zp_ntt_create(arg.log_n, num_buffers, &plan_), "NttGpuDriver");
     for (unsigned i = 0; i < zp_ntt_max_buffers_in_flight(plan_); i++) buffers_.push_back(UserBuffer(plan_, (int)i));
   }
   ~NttGpuDriver() { zp_ntt_destroy(plan_); }

@@ -52,6 +52,7 @@ def lib():
         L.orc_ntt_inplace_unscaled.argtypes = [vp, ctypes.c_uint, ctypes.c_int]
         L.orc_ntt_four_step.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
         L.orc_ntt_many.argtypes = [vp, ctypes.c_uint, ctypes.c_size_t, ctypes.c_int, ctypes.c_int]
+        L.orc_ntt_reverse.argtypes = [vp, vp, ctypes.c_uint]
         for name in ("orc_fq_mul", "orc_fq_add", "orc_fq_sub"):
             getattr(L, name).argtypes = [vp, vp, vp]
         for name in ("orc_fq_inv", "orc_fq_to_mont", "orc_fq_from_mont"):
@@ -117,6 +118,15 @@ def ntt(a, inverse=False, scaled=True):
     else:
         lib().orc_ntt_inplace_unscaled(_ptr(x), log_n, 1 if inverse else 0)
     return x
+
+
+def ntt_reverse(a, log_row_size):
+    """The REVERSE debug core's output for a (4^log_row_size uint64)."""
+    src = np.ascontiguousarray(a, dtype=np.uint64)
+    assert len(src) == 1 << (2 * log_row_size)
+    dst = np.empty_like(src)
+    lib().orc_ntt_reverse(_ptr(dst), _ptr(src), log_row_size)
+    return dst
 
 
 def ntt_four_step(a, log_rows):

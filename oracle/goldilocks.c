@@ -191,3 +191,40 @@ void orc_ntt_many(uint64_t *a, unsigned log_n, size_t count, int dir, int thread
   }
   for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
 }
+
+/* ---- REVERSE core model (zprize/ntt/host/reference.cpp:7-84): the FPGA
+ * build's memory-path debug core.  Same two phases as the 4-step transform
+ * with each row DFT replaced by reversing the row (do_stuff, :18-21):
+ * transpose, reverse rows, transpose (phase 1, :69-74); multiply element
+ * (i, j) by w^(i j), w = OMEGA[2 log_row_size] (slow_apply_twiddle, :46-57);
+ * reverse rows, transpose (phase 2, :79-83). */
+static void rev_transpose(uint64_t *p, uint64_t r) { /* reference.cpp:7-15 */
+  for (uint64_t i = 0; i < r; i++)
+    for (uint64_t j = i + 1; j < r; j++) {
+      uint64_t t = p[i * r + j];
+      p[i * r + j] = p[j * r + i];
+      p[j * r + i] = t;
+    }
+}
+
+static void rev_rows(uint64_t *p, uint64_t r) {
+  for (uint64_t i = 0; i < r; i++)
+    for (uint64_t a = 0, b = r - 1; a < b; a++, b--) {
+      uint64_t t = p[i * r + a];
+      p[i * r + a] = p[i * r + b];
+      p[i * r + b] = t;
+    }
+}
+
+void orc_ntt_reverse(uint64_t *dst, const uint64_t *src, unsigned log_row_size) {
+  const uint64_t r = 1ull << log_row_size;
+  memcpy(dst, src, sizeof(uint64_t) * r * r);
+  rev_transpose(dst, r);
+  rev_rows(dst, r);
+  rev_transpose(dst, r);
+  const uint64_t w = orc_gl_root(2 * log_row_size);
+  for (uint64_t i = 0; i < r; i++)
+    for (uint64_t j = 0; j < r; j++) dst[i * r + j] = orc_gl_mul(dst[i * r + j], orc_gl_pow(w, i * j));
+  rev_rows(dst, r);
+  rev_transpose(dst, r);
+}

@@ -43,6 +43,9 @@ void orc_ntt_inplace_unscaled(uint64_t *a, unsigned log_n, int dir);
 void orc_ntt_four_step(uint64_t *a, unsigned log_n, unsigned log_rows);
 /* Many independent transforms (for throughput baselines); threads <= 0 uses 1. */
 void orc_ntt_many(uint64_t *a, unsigned log_n, size_t count, int dir, int threads);
+/* The REVERSE debug core of the FPGA NTT build (zprize/ntt/host/reference.cpp:60-84):
+ * n = 4^log_row_size elements, src and dst distinct. */
+void orc_ntt_reverse(uint64_t *dst, const uint64_t *src, unsigned log_row_size);
 
 /* ---------------------------------------------------------------- MSM --- */
 

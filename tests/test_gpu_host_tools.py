@@ -3,7 +3,10 @@ host tool built beside the library."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
+
+import oracle
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -25,3 +28,83 @@ def test_host_tool(args):
 def test_host_tool_rejects_unknown_flag():
     r = subprocess.run([TOOL, "test", "--bogus", "1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
+
+
+def _write_hex(path, vals, style):
+    """'padded': one 0x%016x per line (the tool's output form); 'mixed': bare hex
+    and 0x-prefixed tokens, several per line (what operator>> std::hex accepts)."""
+    if style == "padded":
+        text = "".join(f"0x{int(v):016x}\n" for v in vals)
+    else:
+        text = "".join((f"{int(v):x} " if i % 3 else f"0x{int(v):x}\n") for i, v in enumerate(vals))
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def _read_hex(path):
+    return np.array([int(t, 16) for t in open(path).read().split()], dtype=np.uint64)
+
+
+@pytest.mark.parametrize("core,log_n,inverse", [("NTT", 12, False), ("NTT", 16, True), ("NTT", 24, False),
+                                                ("REVERSE", 10, False)])
+def test_evaluate_given_flow(tmp_path, core, log_n, inverse):
+    """evaluate_given.cpp: hex text in (with or without 0x), "0x%016x" lines out,
+    checked against the oracle; --expected-filename reports PASSED / FAILED."""
+    print("evaluate", core, log_n, inverse, flush=True)
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n)
+    x = rng.integers(0, oracle.P, size=n, dtype=np.uint64)
+    inp, out, exp = tmp_path / "in.txt", tmp_path / "out.txt", tmp_path / "exp.txt"
+    _write_hex(inp, x, "mixed")
+    if core == "REVERSE":
+        want = oracle.ntt_reverse(x, log_n // 2)
+        size = ["--core-type", "REVERSE", "--log-row-size", str(log_n // 2)]
+    else:
+        want = oracle.ntt(x, inverse=inverse)
+        size = ["--log-n", str(log_n)]
+    _write_hex(exp, want, "padded")
+    args = [TOOL, "evaluate"] + size + ["--input-filename", str(inp), "--output-filename", str(out),
+                                       "--expected-filename", str(exp), "--xclbin", "unused"]
+    if inverse:
+        args.append("--inverse")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "EVALUATE PASSED" in r.stdout
+    lines = open(out).read().splitlines()
+    assert len(lines) == n and all(len(l) == 18 and l.startswith("0x") for l in lines)
+    assert (_read_hex(out) == want).all()
+    bad = want.copy()
+    bad[n // 3] ^= np.uint64(1)
+    _write_hex(exp, bad, "padded")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "EVALUATE FAILED" in r.stdout and f"i = {n // 3}" in r.stdout
+
+
+def test_evaluate_input_count_errors(tmp_path):
+    """evaluate_given.cpp:36-55: exactly n values, else an error naming the count."""
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    for count, msg in [(15, "but only got 15?"), (17, "but got more?")]:
+        _write_hex(inp, np.arange(count, dtype=np.uint64), "padded")
+        r = subprocess.run([TOOL, "evaluate", "--log-n", "4", "--input-filename", str(inp), "--output-filename",
+                            str(out)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and msg in r.stderr
+    r = subprocess.run([TOOL, "evaluate", "--log-n", "4", "--input-filename", str(tmp_path / "missing"),
+                        "--output-filename", str(out)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Failed to open input_file" in r.stderr
+
+
+@pytest.mark.parametrize("args", [["test", "--core-type", "REVERSE", "--log-row-size", "6", "--num-test-cases", "2"],
+                                  ["test", "--core-type", "REVERSE", "--log-row-size", "12", "--num-test-cases", "1"]])
+def test_reverse_core_test_mode(args):
+    print("host tool", args, flush=True)
+    r = subprocess.run([TOOL] + args, capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "NTT TEST PASSED" in r.stdout
+
+
+def test_usage_errors():
+    for args, msg in [(["test", "--core-type", "REVERSE"], "--log-row-size must be specified"),
+                      (["test", "--memory-layout", "OPTIMIZED_LAYOUT"], "OPTIMIZED_LAYOUT is not supported"),
+                      (["test", "--core-type", "FFT"], "Unknown core type")]:
+        r = subprocess.run([TOOL] + args, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and msg in r.stderr

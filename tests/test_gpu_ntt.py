@@ -155,3 +155,43 @@ def test_batch_graph_replays():
         b.close()
         for p in ptrs:
             call("zp_device_free", ctypes.c_void_p(p))
+
+
+@pytest.mark.parametrize("log_r", list(range(1, 12)))
+def test_reverse_core_vs_oracle(log_r):
+    """The REVERSE debug core (NttFpgaDriverArg::create_reverse) against the
+    oracle's step-by-step restatement of reference.cpp:60-84."""
+    _say(f"reverse core log_row_size={log_r}")
+    x = _rand(1 << (2 * log_r), 300 + log_r)
+    with zntt.NttGpuDriver.reverse(log_r) as d:
+        assert d.input_vector_size() == 1 << (2 * log_r)
+        assert np.array_equal(d.simple_evaluate(x), oracle.ntt_reverse(x, log_r))
+
+
+def test_reverse_core_2p24_device_and_errors():
+    """log_row_size 12 (2^24, the reference's size) through the device entry point,
+    spot-checked with the element formula out[i][j] = in[R-1-j][R-1-i] w^(j (R-1-i));
+    the per-pass timing works on the REVERSE plan; an inverse is rejected."""
+    from hardcaml_zprize_amd.msm import DeviceBuffer
+
+    _say("reverse core 2^24")
+    log_r, R = 12, 1 << 12
+    x = _rand(R * R, 77)
+    dev = DeviceBuffer(x.nbytes).upload(x)
+    try:
+        with zntt.NttGpuDriver.reverse(log_r) as d:
+            d.inplace_device(dev.ptr)
+            call("zp_device_synchronize")
+            got = dev.download(np.empty_like(x))
+            w = pow(7, (P - 1) >> 24, P)
+            rng = np.random.default_rng(5)
+            for i, j in rng.integers(0, R, size=(2000, 2)):
+                i, j = int(i), int(j)
+                want = int(x[(R - 1 - j) * R + (R - 1 - i)]) * pow(w, j * (R - 1 - i), P) % P
+                assert int(got[i * R + j]) == want
+            m1, m2 = d.time_passes(dev.ptr, reps=2)
+            assert m1 > 0 and m2 > 0
+            with pytest.raises(zntt.ZpError):
+                d.inplace_device(dev.ptr, zntt.INVERSE)
+    finally:
+        dev.free()

@@ -139,3 +139,19 @@ def test_planted_mistake_is_caught():
     out = oracle.ntt(a)
     w_wrong = oracle.gl_root(5)
     assert any(int(out[k]) != (1 + 2 * pow(w_wrong, k, P)) % P for k in range(16))
+
+
+@pytest.mark.parametrize("log_r", [1, 2, 3, 4, 5])
+def test_reverse_core_closed_form(log_r):
+    """orc_ntt_reverse (reference.cpp:60-84 restated step by step) against the
+    element formula its steps compose to: with R = 2^log_r and w = OMEGA[2 log_r],
+    out[i][j] = in[R-1-j][R-1-i] * w^(j (R-1-i))."""
+    R = 1 << log_r
+    rng = np.random.default_rng(log_r)
+    a = rng.integers(0, P, size=R * R, dtype=np.uint64)
+    got = oracle.ntt_reverse(a, log_r)
+    w = pow(7, (P - 1) >> (2 * log_r), P)
+    for i in range(R):
+        for j in range(R):
+            want = int(a[(R - 1 - j) * R + (R - 1 - i)]) * pow(w, j * (R - 1 - i), P) % P
+            assert int(got[i * R + j]) == want
