@@ -355,19 +355,28 @@ struct zp_msm_workspace {
   WinDev *wins = nullptr;
   uint8_t *scan_tmp = nullptr;
   std::vector<Xyzt> h_winsum;
+  uint64_t *nc_sums = nullptr;  // 5 words per exceptional-point group per job
+  size_t cap_nc = 0;
+  std::vector<uint64_t> h_nc_sums;
 
   void release() {
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
-                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs};
+                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
   }
 };
 
-struct NonConv {
-  uint64_t idx;
-  F x, y;  // Weierstrass affine, R' Montgomery form
+// Table points without a twisted-Edwards image (driver.cpp:160-172), grouped
+// by value: the curve has only five such points, however many times a table
+// repeats them.  Their digits are skipped by the bucket pipeline (their
+// precomputed entry is the identity); per batch k_nc_sums adds up each
+// group's scalars exactly on the GPU and the host adds S_g * P_g once per
+// group (driver.cpp:349-372 does one double-and-add per point).
+struct NcGroup {
+  F x, y;                     // Weierstrass affine, R' Montgomery form, canonical
+  std::vector<uint32_t> idx;  // table indices, ascending
 };
 
 struct zp_msm_context {
@@ -376,7 +385,10 @@ struct zp_msm_context {
   hipStream_t stream = nullptr;
   PackedPre *d_pts = nullptr;
   uint8_t *d_conv = nullptr;
-  std::vector<NonConv> nonconv;
+  std::vector<NcGroup> nc_groups;
+  size_t num_nonconv = 0;
+  uint32_t *d_nc_idx = nullptr;  // every group's indices, group after group
+  uint32_t *d_nc_off = nullptr;  // nc_groups.size() + 1 offsets into d_nc_idx
   zp_msm_workspace ws;
   uint64_t *d_scalars = nullptr;  // staging for host scalars
   size_t cap_scalars = 0;
@@ -400,11 +412,50 @@ static uint32_t auto_log_g(uint64_t K, int requested) {
   return lg;
 }
 
+// Exact 320-bit sums of the scalars of each exceptional-point group (one
+// block per group), over the group's table indices below the job's prefix n.
+__global__ void __launch_bounds__(256) k_nc_sums(const uint64_t *scalars, const uint32_t *idx,
+                                                 const uint32_t *goff, uint64_t n, uint64_t *out) {
+  __shared__ uint64_t red[256][5];
+  const uint32_t g = blockIdx.x, tid = threadIdx.x;
+  uint64_t acc[5] = {0, 0, 0, 0, 0};
+  for (uint32_t e = goff[g] + tid; e < goff[g + 1]; e += 256) {
+    const uint32_t i = idx[e];
+    if (i >= n) continue;
+    uint64_t c = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      const uint64_t a = acc[w] + c;
+      const uint64_t c1 = a < c;
+      acc[w] = a + scalars[4 * (uint64_t)i + w];
+      c = c1 + (acc[w] < a);
+    }
+    acc[4] += c;
+  }
+#pragma unroll
+  for (int w = 0; w < 5; w++) red[tid][w] = acc[w];
+  __syncthreads();
+  for (uint32_t h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      uint64_t c = 0;
+#pragma unroll
+      for (int w = 0; w < 5; w++) {
+        const uint64_t a = red[tid][w] + c;
+        const uint64_t c1 = a < c;
+        red[tid][w] = a + red[tid + h][w];
+        c = c1 + (red[tid][w] < a);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < 5) out[5 * (uint64_t)g + tid] = red[0][tid];
+}
+
 // Window combine (driver.cpp:340-347 as Horner), conversion to affine
 // Weierstrass (bls12_377_g1.h:427-439) and the non-convertible fallback
 // (driver.cpp:349-372), then the wire form (x R, y R, R) or z = 0.
-static void finalize(const HostLayout &L, const Xyzt *win, const std::vector<NonConv> &nonconv, size_t n,
-                     const biginteger256_t *nc_scalars, g1_projective_t *out) {
+static void finalize(const HostLayout &L, const Xyzt *win, const std::vector<NcGroup> &groups,
+                     const uint64_t *nc_sums, g1_projective_t *out) {
   const FConsts &K = consts();
   Xyzt acc = win[L.W - 1];
   for (int w = L.W - 2; w >= 0; w--) {
@@ -425,17 +476,25 @@ static void finalize(const HostLayout &L, const Xyzt *win, const std::vector<Non
     res.Y = f_canonical(f_mul(v, K.inv_cb));
     res.Z = K.one;
   }
-  size_t c = 0;
-  for (const NonConv &nc : nonconv) {
-    if (nc.idx >= n) continue;
-    const biginteger256_t &k = nc_scalars[c++];
+  // S_g * P_g for each group of exceptional points: S_g is the plain integer
+  // sum of the group's scalars (no reduction mod r: these points lie outside
+  // the order-r subgroup), up to 2^288.
+  for (size_t g = 0; g < groups.size(); g++) {
+    const uint64_t *S = nc_sums + 5 * g;
+    int top = -1;
+    for (int bit = 319; bit >= 0; bit--)
+      if ((S[bit >> 6] >> (bit & 63)) & 1) {
+        top = bit;
+        break;
+      }
+    if (top < 0) continue;
     Jac kp;
     kp.X = f_zero();
     kp.Y = K.one;
     kp.Z = f_zero();
-    for (int bit = 252; bit >= 0; bit--) {
+    for (int bit = top; bit >= 0; bit--) {
       jac_dbl(kp, K);
-      if ((k.data[bit >> 6] >> (bit & 63)) & 1) jac_add_aff(kp, nc.x, nc.y, K);
+      if ((S[bit >> 6] >> (bit & 63)) & 1) jac_add_aff(kp, groups[g].x, groups[g].y, K);
     }
     jac_add_jac(res, kp, K);
   }
@@ -456,7 +515,6 @@ struct HostJob {
   zp_msm_context *ctx;
   size_t n;
   const uint64_t *d_scalars;
-  const biginteger256_t *h_scalars;  // optional host copy (for the fallback points)
   g1_projective_t *out;
   int window = 0;  // per-job window override (autotuner), 0 = none
 };
@@ -649,23 +707,28 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(hipGetLastError());
   ws.h_winsum.resize(Wt);
   ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
+  // exceptional-point scalar sums, per job and group
+  std::vector<size_t> nc0(count + 1, 0);
+  for (size_t j = 0; j < count; j++) nc0[j + 1] = nc0[j] + jobs[j].ctx->nc_groups.size();
+  if (nc0[count]) {
+    ZP_TRY(grow(ws.nc_sums, ws.cap_nc, 5 * nc0[count]));
+    for (size_t j = 0; j < count; j++) {
+      const zp_msm_context *c = jobs[j].ctx;
+      if (c->nc_groups.empty()) continue;
+      hipLaunchKernelGGL(k_nc_sums, dim3((unsigned)c->nc_groups.size()), dim3(256), 0, stream, jobs[j].d_scalars,
+                         c->d_nc_idx, c->d_nc_off, (uint64_t)jobs[j].n, ws.nc_sums + 5 * nc0[j]);
+      ZP_HIP(hipGetLastError());
+    }
+    ws.h_nc_sums.resize(5 * nc0[count]);
+    ZP_HIP(hipMemcpyAsync(ws.h_nc_sums.data(), ws.nc_sums, 5 * nc0[count] * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream));
+  }
   ZP_HIP(hipStreamSynchronize(stream));
-  // ---- host finalisation (window combine, conversion, fallback points)
-  std::vector<int> status(count, ZP_OK);
+  // ---- host finalisation (window combine, conversion, exceptional points)
   auto fin = [&](size_t j) {
     const HostJob &h = jobs[j];
-    std::vector<biginteger256_t> ncs;
-    for (const NonConv &nc : h.ctx->nonconv) {
-      if (nc.idx >= h.n) continue;
-      biginteger256_t k{{0, 0, 0, 0}};
-      if (h.h_scalars) {
-        k = h.h_scalars[nc.idx];
-      } else if (hipMemcpy(&k, h.d_scalars + 4 * nc.idx, 32, hipMemcpyDeviceToHost) != hipSuccess) {
-        status[j] = ZP_ERR_HIP;
-      }
-      ncs.push_back(k);
-    }
-    finalize(layouts[j], ws.h_winsum.data() + jd[j].win0, h.ctx->nonconv, h.n, ncs.data(), h.out);
+    finalize(layouts[j], ws.h_winsum.data() + jd[j].win0, h.ctx->nc_groups, ws.h_nc_sums.data() + 5 * nc0[j],
+             h.out);
   };
   const size_t nthreads = std::min<size_t>(count, 16);
   if (nthreads <= 1) {
@@ -678,9 +741,6 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
         for (size_t j = next++; j < count; j = next++) fin(j);
       });
     for (auto &t : th) t.join();
-  }
-  for (int st : status) {
-    ZP_CHECK(st == ZP_OK, st, "MSM batch: scalar copy for a fallback point failed");
   }
   return ZP_OK;
 }
@@ -801,7 +861,7 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
   if (d_tmp) (void)hipFree(d_tmp);
   if (d_flag) (void)hipFree(d_flag);
   if (st != ZP_OK) return st;
-  // Non-convertible points (driver.cpp:160-172): kept on the host.
+  // Non-convertible points (driver.cpp:160-172), grouped by value.
   std::vector<uint8_t> conv(n);
   if (n) ZP_HIP(hipMemcpy(conv.data(), ctx->d_conv, n, hipMemcpyDeviceToHost));
   for (size_t i = 0; i < n; i++) {
@@ -811,14 +871,31 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
       ZP_HIP(hipMemcpy(&p, points + i, sizeof(p), hipMemcpyDeviceToHost));
     else
       p = points[i];
-    NonConv nc;
-    nc.idx = i;
     U384 x, y;
     memcpy(x.w, p.x.data, 48);
     memcpy(y.w, p.y.data, 48);
-    nc.x = f_canonical(f_mul(f_from_u384(x), to_r));
-    nc.y = f_canonical(f_mul(f_from_u384(y), to_r));
-    ctx->nonconv.push_back(nc);
+    const F fx = f_canonical(f_mul(f_from_u384(x), to_r));
+    const F fy = f_canonical(f_mul(f_from_u384(y), to_r));
+    NcGroup *grp = nullptr;
+    for (NcGroup &g : ctx->nc_groups)
+      if (f_eq(g.x, fx) && f_eq(g.y, fy)) grp = &g;
+    if (!grp) {
+      ctx->nc_groups.push_back(NcGroup{fx, fy, {}});
+      grp = &ctx->nc_groups.back();
+    }
+    grp->idx.push_back((uint32_t)i);
+    ctx->num_nonconv++;
+  }
+  if (!ctx->nc_groups.empty()) {
+    std::vector<uint32_t> all, off{0};
+    for (const NcGroup &g : ctx->nc_groups) {
+      all.insert(all.end(), g.idx.begin(), g.idx.end());
+      off.push_back((uint32_t)all.size());
+    }
+    ZP_HIP(hipMalloc(&ctx->d_nc_idx, all.size() * sizeof(uint32_t)));
+    ZP_HIP(hipMalloc(&ctx->d_nc_off, off.size() * sizeof(uint32_t)));
+    ZP_HIP(hipMemcpy(ctx->d_nc_idx, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ZP_HIP(hipMemcpy(ctx->d_nc_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
   return ZP_OK;
 }
@@ -884,6 +961,8 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
   if (ctx->d_pts) (void)hipFree(ctx->d_pts);
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
+  if (ctx->d_nc_idx) (void)hipFree(ctx->d_nc_idx);
+  if (ctx->d_nc_off) (void)hipFree(ctx->d_nc_off);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -897,7 +976,7 @@ extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t n
   for (uint64_t b = 0; b < num_batches; b++) {
     const biginteger256_t *hs = scalars + b * n;
     if (n) ZP_HIP(hipMemcpyAsync(ctx->d_scalars, hs, n * 32, hipMemcpyHostToDevice, ctx->stream));
-    HostJob job{ctx, n, ctx->d_scalars, hs, out + b};
+    HostJob job{ctx, n, ctx->d_scalars, out + b};
     ZP_TRY(run_batch(ctx->ws, &job, 1, MsmOptions(), ctx->stream));
   }
   return ZP_OK;
@@ -915,7 +994,7 @@ extern "C" int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uin
            ZP_ERR_INVALID_ARG, "zp_msm_mult_device: NULL argument");
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
   for (uint64_t b = 0; b < num_batches; b++) {
-    HostJob job{ctx, ctx->n, reinterpret_cast<const uint64_t *>(d_scalars + b * ctx->n), nullptr, out + b};
+    HostJob job{ctx, ctx->n, reinterpret_cast<const uint64_t *>(d_scalars + b * ctx->n), out + b};
     ZP_TRY(run_batch(ctx->ws, &job, 1, MsmOptions(), s));
   }
   return ZP_OK;
@@ -927,7 +1006,7 @@ extern "C" int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_proje
   ZP_CHECK(n <= ctx->n, ZP_ERR_TOO_LARGE, "zp_msm_mult_prefix_device: n %zu > %zu points", n, ctx->n);
   ZP_CHECK(d_scalars != nullptr || n == 0, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_device: NULL scalars");
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  HostJob job{ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), nullptr, out};
+  HostJob job{ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), out};
   return run_batch(ctx->ws, &job, 1, MsmOptions(), s);
 }
 
@@ -967,13 +1046,13 @@ extern "C" int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs,
     ZP_CHECK(J.d_scalars != nullptr || J.n == 0, ZP_ERR_INVALID_ARG, "job %zu: NULL scalars", j);
     ZP_CHECK(J.window == 0 || (J.window >= 8 && J.window <= 24), ZP_ERR_INVALID_ARG, "job %zu: window %d", j,
              J.window);
-    hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), nullptr, J.out, J.window};
+    hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), J.out, J.window};
   }
   return run_batch(*ws, hj.data(), count, o, (hipStream_t)stream);
 }
 
 extern "C" size_t zp_msm_num_points(const zp_msm_context *ctx) { return ctx ? ctx->n : 0; }
-extern "C" size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx) { return ctx ? ctx->nonconv.size() : 0; }
+extern "C" size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx) { return ctx ? ctx->num_nonconv : 0; }
 
 // ---------------------------------------------------------- generator --
 static std::mutex g_gen_mu;

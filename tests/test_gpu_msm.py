@@ -261,3 +261,38 @@ def test_batch_mixed_jobs_two_tables():
             b.free()
         for _, ctx in tables:
             ctx.close()
+
+
+def test_many_exceptional_points(msm_golden):
+    """A table that repeats the curve's points without a twisted-Edwards image
+    thousands of times (grouped on the device: one exact scalar sum per group),
+    interleaved with generator multiples and infinity; several batches, both
+    entry points, and prefix jobs that cut through the repeated points.  The
+    oracle restates the reference's per-point host fallback (driver.cpp:349-372)."""
+    _say("msm many exceptional points")
+    names = {c["name"]: c for c in msm_golden["msm_cases"]}
+    special = [oracle.affine_from_json(p) for p in names["unit_intermediate_infinity_1"]["points"]]
+    rng = random.Random(33)
+    n, batches = 6000, 3
+    pts = zmsm.generate_points(_words([rng.randrange(1, R) for _ in range(n)], 4))
+    picks = rng.sample(range(n), 3000)
+    for t, i in enumerate(picks):
+        pts[i] = special[t % 3]
+    pts[5] = np.zeros(1, dtype=zmsm.G1_AFFINE_DTYPE)[0]
+    pts[5]["infinity"] = 1
+    sc = _scalars(rng, n * batches)
+    sc[picks[0:300]] = _words([R - 1] * 300, 4)  # large group sums (carries into the fifth word)
+    with zmsm.MsmContext(pts) as ctx:
+        assert ctx.num_nonconvertible() == 3000
+        out = ctx.mult(sc, batches)
+        exp = oracle.msm(pts, sc, batches, threads=8)
+        assert out.tobytes() == exp.tobytes()
+        assert ctx.mult_abi(sc, batches).tobytes() == exp.tobytes()
+        d = zmsm.DeviceBuffer(sc.nbytes).upload(sc)
+        try:
+            for m in (1, 7, 1500, 4321, n):
+                got = ctx.mult_prefix_device(m, d.ptr)
+                want = oracle.msm(pts[:m], sc[:m], 1, threads=8)[0]
+                assert got.tobytes() == want.tobytes(), m
+        finally:
+            d.free()
