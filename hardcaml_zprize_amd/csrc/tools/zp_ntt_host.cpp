@@ -10,8 +10,9 @@
 //                        (evaluate_given.cpp:19-76); --expected-filename adds a
 //                        diff against a file in the same format.
 // Flags: --log-n K | --log-row-size R (n = 4^R, the reference's form),
-// --core-type NTT|REVERSE, --memory-layout NORMAL_LAYOUT, --xclbin (accepted,
-// unused), --num-test-cases T, --num-evaluations E, --input-filename,
+// --core-type NTT|REVERSE, --memory-layout NORMAL_LAYOUT|OPTIMIZED_LAYOUT and
+// --log-blocks B (the FPGA's device layout and core count: accepted, results
+// identical), --xclbin (accepted, unused), --num-test-cases T, --num-evaluations E, --input-filename,
 // --output-filename, --expected-filename, --inverse.
 #include <algorithm>
 #include <chrono>
@@ -99,9 +100,10 @@ static Args parse(int argc, char **argv) {
       else if (v != "NTT")
         throw UsageError("Unknown core type " + v);
     } else if (f == "--memory-layout") {
-      if (v == "OPTIMIZED_LAYOUT")
-        throw UsageError("OPTIMIZED_LAYOUT is not supported: the GPU transform works on the natural layout");
-      if (v != "NORMAL_LAYOUT") throw UsageError("Unknown memory layout " + v);
+      // the FPGA driver's device-buffer layout; vectors are natural order in both
+      if (v != "NORMAL_LAYOUT" && v != "OPTIMIZED_LAYOUT") throw UsageError("Unknown memory layout " + v);
+    } else if (f == "--log-blocks") {
+      (void)std::stoull(v);  // the FPGA's core count; no GPU meaning
     } else if (f == "--xclbin") {
       // the device is ready without a binary; accepted for command-line compatibility
     } else if (f == "--num-test-cases") {

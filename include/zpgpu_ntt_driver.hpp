@@ -15,18 +15,38 @@
 
 namespace zpgpu {
 
-// NttFpgaDriverArg::create_ntt_2_24 / _2_18 / _2_12 (driver.h:36-56): the
-// memory layout and log_blocks arguments of the FPGA build have no GPU
-// meaning and are not taken; the transform size is.
-// create_reverse (driver.cpp:100-104) selects the REVERSE debug core.
+// driver.h:24-27.  The FPGA driver permutes vectors into OPTIMIZED_LAYOUT
+// device buffers and back (driver.cpp:251-277, ntt_preprocessing.cpp,
+// ntt_postprocessing.cpp); callers hand it and get back natural-order
+// vectors in both layouts.  The GPU transform works on the natural layout
+// directly, so both layouts behave identically here.
+enum class MemoryLayout { NORMAL_LAYOUT, OPTIMIZED_LAYOUT };
+
+// NttFpgaDriverArg::create_ntt_2_24 / _2_18 / _2_12 / create_reverse
+// (driver.h:36-56), with the same arguments; log_blocks sizes the FPGA's
+// core array and has no GPU meaning.  create(log_n) takes any size 2^1..2^24.
 struct NttGpuDriverArg {
   unsigned log_n;
   bool reverse = false;
-  static NttGpuDriverArg create_ntt_2_24() { return {24}; }
-  static NttGpuDriverArg create_ntt_2_18() { return {18}; }
-  static NttGpuDriverArg create_ntt_2_12() { return {12}; }
+  MemoryLayout memory_layout = MemoryLayout::NORMAL_LAYOUT;
+  uint64_t log_blocks = 0;
+  static NttGpuDriverArg create_ntt_2_24(MemoryLayout l = MemoryLayout::NORMAL_LAYOUT, uint64_t log_blocks = 0) {
+    return {24, false, l, log_blocks};
+  }
+  static NttGpuDriverArg create_ntt_2_18(MemoryLayout l = MemoryLayout::NORMAL_LAYOUT, uint64_t log_blocks = 0) {
+    return {18, false, l, log_blocks};
+  }
+  static NttGpuDriverArg create_ntt_2_12(MemoryLayout l = MemoryLayout::NORMAL_LAYOUT, uint64_t log_blocks = 0) {
+    return {12, false, l, log_blocks};
+  }
+  static NttGpuDriverArg create_reverse(MemoryLayout l, uint64_t log_row_size, uint64_t log_blocks = 0) {
+    return {(unsigned)(2 * log_row_size), true, l, log_blocks};
+  }
+  static NttGpuDriverArg create_reverse(unsigned log_row_size) {
+    return create_reverse(MemoryLayout::NORMAL_LAYOUT, log_row_size);
+  }
   static NttGpuDriverArg create(unsigned log_n) { return {log_n}; }
-  static NttGpuDriverArg create_reverse(unsigned log_row_size) { return {2 * log_row_size, true}; }
+  uint64_t row_size() const { return 1ull << (log_n / 2); }
   uint64_t num_elements() const { return 1ull << log_n; }
 };
 

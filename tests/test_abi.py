@@ -50,3 +50,33 @@ def test_reference_symbol_names_present():
     lib = ctypes.CDLL(_lib.LIB_PATH)
     for n in ("msm_init", "msm_mult"):
         assert hasattr(lib, n)
+
+
+def test_cpp_mirror_takes_reference_call_forms(tmp_path):
+    """include/zpgpu_ntt_driver.hpp compiles with the reference programs' own
+    NttFpgaDriverArg calls (driver.h:36-56, test_random.cpp, evaluate_given.cpp)."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if not gxx:
+        pytest.skip("g++ not installed")
+    src = tmp_path / "calls.cpp"
+    src.write_text("""
+#include "zpgpu_ntt_driver.hpp"
+using zpgpu::MemoryLayout;
+using zpgpu::NttGpuDriverArg;
+int main() {
+  NttGpuDriverArg a = NttGpuDriverArg::create_ntt_2_24(MemoryLayout::OPTIMIZED_LAYOUT, 3);
+  NttGpuDriverArg b = NttGpuDriverArg::create_ntt_2_18(MemoryLayout::NORMAL_LAYOUT, 0);
+  NttGpuDriverArg c = NttGpuDriverArg::create_ntt_2_12(MemoryLayout::NORMAL_LAYOUT, 0);
+  NttGpuDriverArg d = NttGpuDriverArg::create_reverse(MemoryLayout::NORMAL_LAYOUT, 12, 0);
+  return (a.num_elements() == (1ull << 24) && b.row_size() == 512 && c.num_elements() == 4096 &&
+          d.num_elements() == (1ull << 24) && d.reverse) ? 0 : 1;
+}
+""")
+    inc = os.path.join(ROOT, "include")
+    subprocess.run([gxx, "-std=c++17", "-I", inc, str(src), "-o", str(tmp_path / "calls"), "-L",
+                    os.path.join(ROOT, "hardcaml_zprize_amd"), "-lzpgpu", "-Wl,-rpath," +
+                    os.path.join(ROOT, "hardcaml_zprize_amd")], check=True, timeout=120)
+    assert subprocess.run([str(tmp_path / "calls")], timeout=60).returncode == 0

@@ -45,11 +45,15 @@ def _read_hex(path):
     return np.array([int(t, 16) for t in open(path).read().split()], dtype=np.uint64)
 
 
-@pytest.mark.parametrize("core,log_n,inverse", [("NTT", 12, False), ("NTT", 16, True), ("NTT", 24, False),
-                                                ("REVERSE", 10, False)])
-def test_evaluate_given_flow(tmp_path, core, log_n, inverse):
+@pytest.mark.parametrize("core,log_n,inverse,layout", [("NTT", 12, False, "NORMAL_LAYOUT"),
+                                                       ("NTT", 16, True, "OPTIMIZED_LAYOUT"),
+                                                       ("NTT", 24, False, "OPTIMIZED_LAYOUT"),
+                                                       ("REVERSE", 10, False, "NORMAL_LAYOUT")])
+def test_evaluate_given_flow(tmp_path, core, log_n, inverse, layout):
     """evaluate_given.cpp: hex text in (with or without 0x), "0x%016x" lines out,
-    checked against the oracle; --expected-filename reports PASSED / FAILED."""
+    checked against the oracle; --expected-filename reports PASSED / FAILED.  The
+    FPGA's OPTIMIZED_LAYOUT (an internal device-buffer layout) gives the same
+    natural-order vectors."""
     print("evaluate", core, log_n, inverse, flush=True)
     n = 1 << log_n
     rng = np.random.default_rng(log_n)
@@ -64,7 +68,8 @@ def test_evaluate_given_flow(tmp_path, core, log_n, inverse):
         size = ["--log-n", str(log_n)]
     _write_hex(exp, want, "padded")
     args = [TOOL, "evaluate"] + size + ["--input-filename", str(inp), "--output-filename", str(out),
-                                       "--expected-filename", str(exp), "--xclbin", "unused"]
+                                       "--expected-filename", str(exp), "--xclbin", "unused",
+                                       "--memory-layout", layout, "--log-blocks", "3"]
     if inverse:
         args.append("--inverse")
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
@@ -104,7 +109,7 @@ def test_reverse_core_test_mode(args):
 
 def test_usage_errors():
     for args, msg in [(["test", "--core-type", "REVERSE"], "--log-row-size must be specified"),
-                      (["test", "--memory-layout", "OPTIMIZED_LAYOUT"], "OPTIMIZED_LAYOUT is not supported"),
+                      (["test", "--memory-layout", "TILED"], "Unknown memory layout"),
                       (["test", "--core-type", "FFT"], "Unknown core type")]:
         r = subprocess.run([TOOL] + args, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and msg in r.stderr
