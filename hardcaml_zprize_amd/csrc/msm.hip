@@ -528,7 +528,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   std::vector<WinDev> wins;
   std::vector<uint32_t> pt0(count), b0(count), gstart;
   std::vector<SegDev> segs;
-  uint64_t K = 0, entries = 0, points = 0, counters = 0;
+  uint64_t K = 0, entries = 0, points = 0, counters = 0, n_max = 0;
   uint32_t max_nb = 0;
   for (size_t j = 0; j < count; j++) {
     const HostJob &h = jobs[j];
@@ -576,6 +576,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       for (uint32_t c = 0; c < d.chunks; c++) segs.push_back(SegDev{(uint32_t)j, d.win0 + (uint32_t)w, c});
     points += h.n;
     entries += (uint64_t)h.n * L.W;
+    n_max = std::max<uint64_t>(n_max, h.n);
   }
   // The LDS histogram holds one window's buckets (<= 2^15 u32 = 128 KiB).
   const char *force_atomic = getenv("ZP_MSM_ATOMIC_SORT");
@@ -690,12 +691,21 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   size_t tb = ws.cap_scan;
   ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, (size_t)K + 1,
                                  rocprim::plus<uint32_t>(), stream));
-  hipLaunchKernelGGL(k_fill_units, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.nunits, ws.ustart,
-                     (uint32_t)K, ws.ubucket);
+  const uint32_t *total_units = ws.ustart + K;  // the scan's last entry
+  hipLaunchKernelGGL(k_fill_units, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.ustart, (uint32_t)K,
+                     total_units, ws.ubucket);
   ZP_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_units, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream, B,
-                     ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, ws.ustart + K, log_u, ws.partial, KC);
+                     ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, total_units, log_u, ws.partial, KC);
   ZP_HIP(hipGetLastError());
+  // A bucket holds at most n_max entries (one digit per point and window),
+  // so at most ceil(n_max / U) units: that many tree levels suffice.
+  const uint64_t units_per_bucket = (n_max + (1ull << log_u) - 1) >> log_u;
+  for (uint64_t stride = 1; stride < units_per_bucket; stride <<= 1) {
+    hipLaunchKernelGGL(k_unit_tree, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ubucket,
+                       ws.ustart, ws.nunits, total_units, (uint32_t)stride, KC);
+    ZP_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.nunits,
                      (uint32_t)K, ws.buckets, KC);
   ZP_HIP(hipGetLastError());

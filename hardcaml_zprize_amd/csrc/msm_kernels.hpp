@@ -13,7 +13,8 @@
 //   k_accumulate     one thread per unit: mixed twisted-Edwards additions of
 //                    the unit's points into an extended accumulator
 //                    (the FPGA bucket pipeline, top.ml / mixed_add_precompute.ml);
-//   k_bucket_sum     per bucket: sum of its unit partials;
+//   k_unit_tree      per bucket: log-depth pairwise sum of its unit partials;
+//   k_bucket_sum     per bucket: the reduced partial (identity if empty);
 //   k_window_groups  per group of g buckets: running-sum reduction
 //                    (driver.cpp:318-339, pippenger.h:25-33) done as
 //                    sum_b (b - lo + 1) S_b + lo * sum_b S_b;
@@ -233,10 +234,23 @@ __global__ void k_units_per_bucket(const uint32_t *cnt, uint32_t K, uint32_t log
     nunits[k] = k < K ? (cnt[k] + (1u << log_u) - 1) >> log_u : 0;
 }
 
-__global__ void k_fill_units(const uint32_t *nunits, const uint32_t *ustart, uint32_t K, uint32_t *ubucket) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
-    const uint32_t s0 = ustart[k], nu = nunits[k];
-    for (uint32_t s = 0; s < nu; s++) ubucket[s0 + s] = k;
+// unit -> bucket: the bucket k with ustart[k] <= u < ustart[k + 1] (ustart is
+// the exclusive scan of the per-bucket unit counts, K + 1 entries; empty
+// buckets share their start with the next bucket, and upper_bound - 1 picks
+// the last of them, the non-empty one).  One thread per unit, so a bucket
+// with very many units is not written by one thread.
+__global__ void k_fill_units(const uint32_t *ustart, uint32_t K, const uint32_t *total_units, uint32_t *ubucket) {
+  const uint32_t total = *total_units;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = K + 1;  // first index with ustart > u lies in (lo, hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ustart[mid] <= u)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    ubucket[u] = lo;
   }
 }
 
@@ -261,15 +275,35 @@ __global__ void __launch_bounds__(MSM_THREADS) k_accumulate(BatchDev B, const ui
   }
 }
 
+// One level of a segmented pairwise reduction of each bucket's unit partials:
+// unit u (rank r within its bucket) absorbs unit u + stride when r is a
+// multiple of 2 stride.  After ceil(log2(max units per bucket)) levels the
+// bucket's first unit holds its sum, so a bucket whose entries fill many units
+// (equal or structured scalars put every point of a window in one bucket)
+// costs log depth instead of a serial walk by one thread.  Within a level no
+// unit is both read and written.
+__global__ void __launch_bounds__(MSM_THREADS) k_unit_tree(Xyzt *partial, const uint32_t *ubucket,
+                                                           const uint32_t *ustart, const uint32_t *nunits,
+                                                           const uint32_t *total_units, uint32_t stride,
+                                                           FConsts K) {
+  const uint32_t total = *total_units;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
+    const uint32_t b = ubucket[u];
+    const uint32_t r = u - ustart[b];
+    if ((r & (2 * stride - 1)) == 0 && r + stride < nunits[b]) {
+      Xyzt acc = partial[u];
+      ec_add(acc, partial[u + stride], K);
+      partial[u] = acc;
+    }
+  }
+}
+
+// Bucket value = its first unit's partial after the tree levels (identity if empty).
 __global__ void __launch_bounds__(MSM_THREADS) k_bucket_sum(const Xyzt *partial, const uint32_t *ustart,
                                                             const uint32_t *nunits, uint32_t K_total,
                                                             Xyzt *buckets, FConsts K) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K_total; k += gridDim.x * blockDim.x) {
-    const uint32_t nu = nunits[k], s0 = ustart[k];
-    Xyzt acc = nu ? partial[s0] : ec_identity(K);
-    for (uint32_t s = 1; s < nu; s++) ec_add(acc, partial[s0 + s], K);
-    buckets[k] = acc;
-  }
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K_total; k += gridDim.x * blockDim.x)
+    buckets[k] = nunits[k] ? partial[ustart[k]] : ec_identity(K);
 }
 
 // acc = m * p for a small non-negative integer m (double-and-add, MSB first).

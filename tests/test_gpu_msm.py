@@ -296,3 +296,32 @@ def test_many_exceptional_points(msm_golden):
                 assert got.tobytes() == want.tobytes(), m
         finally:
             d.free()
+
+
+def test_hot_buckets_closed_form():
+    """Scalars that put every point of a window into one bucket (all equal, all
+    ones, half zero): the bucket's unit partials are summed by the log-depth
+    tree, not one thread; results equal ((sum s_i k_i) mod r) G."""
+    import time
+
+    rng = random.Random(404)
+    n = 1 << 20
+    ks_int = [rng.randrange(1, R) for _ in range(n)]
+    ks = _words(ks_int, 4)
+    g = oracle.generator()
+    s_eq = rng.randrange(1, R)
+    cases = {
+        "all_equal": np.tile(_words([s_eq], 4), (n, 1)),
+        "all_ones": np.tile(np.array([[1, 0, 0, 0]], dtype=np.uint64), (n, 1)),
+        "half_zero": np.concatenate([np.zeros((n // 2, 4), dtype=np.uint64),
+                                     np.tile(_words([R - 1], 4), (n - n // 2, 1))]),
+    }
+    with zmsm.MsmContext(zmsm.generate_points(ks)) as ctx:
+        for name, sc in cases.items():
+            _say(f"msm hot buckets {name}")
+            t0 = time.perf_counter()
+            out = ctx.mult(sc, 1)
+            dt = time.perf_counter() - t0
+            e = oracle.dot_mod_r(sc, ks)
+            assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(g, e)), name
+            assert dt < 5.0, (name, dt)
