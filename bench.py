@@ -171,11 +171,21 @@ def bench_msm(args, probe):
             dev_ms.append((t3 - t2) * 1e3)
         log(f"MSM call {call_i}: msm_mult {1e3 * (t1 - t0):.1f} ms, resident scalars {1e3 * (t3 - t2):.1f} ms")
     dsc.free()
+    # msm_mult over several batches (the harness's num_batches): the upload of
+    # batch b + 1 overlaps the computation of batch b.
+    batches = 2
+    sc = np.concatenate([rand_scalars(np.random.default_rng([args.seed, 99, b]), n) for b in range(batches)])
+    t0 = time.perf_counter()
+    ctx.mult(sc, batches)
+    multi_ms = (time.perf_counter() - t0) * 1e3 / batches
+    del sc
+    log(f"MSM msm_mult x{batches} batches: {multi_ms:.1f} ms per batch")
     ctx.close()
     ms = float(np.median(dev_ms))
     work = 140 * n * 288  # reference algorithm's v_mad_u64_u32 lane-ops (20 windows x 7 Fq muls x 288 MACs)
     peak = probe["mad_u64_u32_lane_ops_per_s"]
-    return {"n": n, "ms": ms, "ms_msm_mult_host_scalars": float(np.median(host_ms)), "point_gen_s": gen_s,
+    return {"n": n, "ms": ms, "ms_msm_mult_host_scalars": float(np.median(host_ms)),
+            "ms_per_batch_msm_mult_2_batches": multi_ms, "point_gen_s": gen_s,
             "init_s": init_s,
             "roofline": {"bound": "valu", "achieved": work / (ms * 1e-3), "peak": peak, "unit": "mad_u64_u32 lane-op/s",
                          "frac": work / (ms * 1e-3) / peak,

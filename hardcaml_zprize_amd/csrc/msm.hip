@@ -392,6 +392,9 @@ struct zp_msm_context {
   zp_msm_workspace ws;
   uint64_t *d_scalars = nullptr;  // staging for host scalars
   size_t cap_scalars = 0;
+  uint64_t *d_scalars2 = nullptr;  // second buffer: the next batch uploads while one computes
+  size_t cap_scalars2 = 0;
+  hipStream_t copy_stream = nullptr;
 };
 
 namespace zp {
@@ -969,6 +972,8 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->ws.release();
   if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
+  if (ctx->d_scalars2) (void)hipFree(ctx->d_scalars2);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->d_pts) (void)hipFree(ctx->d_pts);
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
   if (ctx->d_nc_idx) (void)hipFree(ctx->d_nc_idx);
@@ -983,11 +988,35 @@ extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t n
            ZP_ERR_INVALID_ARG, "zp_msm_mult: NULL argument");
   const size_t n = ctx->n;
   ZP_TRY(grow(ctx->d_scalars, ctx->cap_scalars, std::max<size_t>(4 * n, 4)));
+  if (num_batches > 1 && n) {
+    ZP_TRY(grow(ctx->d_scalars2, ctx->cap_scalars2, std::max<size_t>(4 * n, 4)));
+    if (!ctx->copy_stream) ZP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  }
+  if (n && num_batches) ZP_HIP(hipMemcpyAsync(ctx->d_scalars, scalars, n * 32, hipMemcpyHostToDevice, ctx->stream));
+  // Batches alternate between two device buffers: while batch b computes on
+  // ctx->stream, a helper thread uploads batch b + 1 on the copy stream
+  // (driver.cpp:495-600 overlaps transfers with work across batches the same
+  // way).  run_batch returns only when batch b is finished, so the buffer the
+  // next upload overwrites is free.
   for (uint64_t b = 0; b < num_batches; b++) {
-    const biginteger256_t *hs = scalars + b * n;
-    if (n) ZP_HIP(hipMemcpyAsync(ctx->d_scalars, hs, n * 32, hipMemcpyHostToDevice, ctx->stream));
-    HostJob job{ctx, n, ctx->d_scalars, out + b};
-    ZP_TRY(run_batch(ctx->ws, &job, 1, MsmOptions(), ctx->stream));
+    uint64_t *cur = (b & 1) ? ctx->d_scalars2 : ctx->d_scalars;
+    uint64_t *nxt = (b & 1) ? ctx->d_scalars : ctx->d_scalars2;
+    int copy_rc = ZP_OK;
+    std::thread copier;
+    if (b + 1 < num_batches && n) {
+      const biginteger256_t *hs = scalars + (b + 1) * n;
+      copier = std::thread([&, hs, nxt] {
+        if (hipMemcpyAsync(nxt, hs, n * 32, hipMemcpyHostToDevice, ctx->copy_stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->copy_stream) != hipSuccess)
+          copy_rc = ZP_ERR_HIP;
+      });
+    }
+    HostJob job{ctx, n, cur, out + b};
+    const int rc = run_batch(ctx->ws, &job, 1, MsmOptions(), ctx->stream);
+    if (copier.joinable()) copier.join();
+    ZP_TRY(rc);
+    ZP_CHECK(copy_rc == ZP_OK, copy_rc, "zp_msm_mult: scalar upload of batch %llu failed",
+             (unsigned long long)(b + 1));
   }
   return ZP_OK;
 }
