@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) NTT figures")
+    ap.add_argument("--host-evals", type=int, default=32)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2^log_n u64 (PMC scaling reference) and exit")
@@ -95,6 +97,58 @@ def bench_ntt(args, torch):
     }
     del dev
     drv.close()
+    return res
+
+
+def bench_ntt_host(args):
+    """The NTT through host buffers, PCIe included: simple_evaluate latency
+    (pageable numpy vectors; bench_latency.cpp's blocking flow) and the buffer
+    API throughput of bench_throughput.cpp:60-107 (every buffer loaded once,
+    E evaluations enqueued round-robin without waiting, then E waits), whose
+    copies overlap the serialized transforms.  Every buffer's output is
+    checked against simple_evaluate of its input."""
+    import numpy as np
+
+    from hardcaml_zprize_amd import ntt as zntt
+
+    n = 1 << args.log_n
+    P = 2**64 - 2**32 + 1
+    rng = np.random.default_rng(args.seed + 1)
+    res = {}
+    with zntt.NttGpuDriver(args.log_n) as drv:
+        x = rng.integers(0, P, size=n, dtype=np.uint64)
+        lat = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            y = drv.simple_evaluate(x)
+            lat.append(time.perf_counter() - t0)
+        lat = sorted(lat[1:])
+        res["simple_evaluate_ms"] = round(lat[len(lat) // 2] * 1e3, 3)
+        nb = drv.max_num_buffers_in_flight()
+        inputs = [rng.integers(0, P, size=n, dtype=np.uint64) for _ in range(nb)]
+        want = [drv.simple_evaluate(a) for a in inputs]
+        bufs = [drv.request_buffer() for _ in range(nb)]
+        for b, a in zip(bufs, inputs):
+            drv.transfer_input_vector_to_user_buffer(b, a)
+        evals = max(args.host_evals, nb)
+        t0 = time.perf_counter()
+        for e in range(evals):
+            drv.enqueue_evaluation_async(bufs[e % nb])
+        for e in range(evals):
+            drv.wait_for_result(bufs[e % nb])
+        secs = time.perf_counter() - t0
+        ok = all(np.array_equal(drv.transfer_user_buffer_to_output_vector(b), w) for b, w in zip(bufs, want))
+        for b in bufs:
+            drv.free_buffer(b)
+        del y
+    if not ok:
+        raise RuntimeError("buffer-API throughput run: an output differs from simple_evaluate")
+    res.update({
+        "buffers": nb, "evaluations": evals,
+        "throughput_ntts_per_s": round(evals / secs, 2),
+        "throughput_gbs_16n": round(evals * 16 * n / secs / 1e9, 2),
+        "basis": "host pinned buffers -> device -> host; 16*n bytes cross PCIe per transform (8n each way)",
+    })
     return res
 
 
@@ -484,6 +538,9 @@ def main():
         "roofline": ntt_roofline(args, ntt),
     }
     out["ntt"] = {"median_ms": ntt["median_ms"], "min_ms": ntt["min_ms"]}
+    if not args.skip_host:
+        out["ntt_host"] = bench_ntt_host(args)
+        log(f"NTT through host buffers: {out['ntt_host']}")
     out["valu_probe"] = probe = probe_rates()
     if not args.skip_msm:
         out["msm"] = bench_msm(args, probe)

@@ -328,13 +328,20 @@ static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_sc
 using namespace zp;
 
 // ------------------------------------------------------------------ plan --
+// A buffer of the asynchronous API: pinned host input/output vectors and a
+// device vector.  An evaluation is copy-in (h2d stream) -> transform (the
+// plan's stream, which serializes transforms: they share the plan's 4-step
+// scratch) -> copy-out (d2h stream), chained by events, so the copies of
+// one buffer overlap the transforms of others.
 struct UserBuffer {
   uint64_t *h_in = nullptr;
   uint64_t *h_out = nullptr;
   uint64_t *d_buf = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t in_done = nullptr;    // copy-in finished
+  hipEvent_t comp_done = nullptr;  // transform finished
+  hipEvent_t done = nullptr;       // copy-out finished (the evaluation's end)
   bool in_use = false;
-  bool enqueued = false;
+  bool enqueued = false;  // evaluated at least once since the request
 };
 
 struct zp_ntt_plan {
@@ -344,7 +351,9 @@ struct zp_ntt_plan {
   NttTables *tables = nullptr;
   uint64_t *d_scratch = nullptr;  // n elements (4-step intermediate)
   uint64_t *d_work = nullptr;     // n elements for simple_evaluate
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // transforms (and simple_evaluate)
+  hipStream_t h2d = nullptr;     // buffer API copy-in
+  hipStream_t d2h = nullptr;     // buffer API copy-out
   std::vector<UserBuffer> buffers;
 };
 
@@ -387,6 +396,8 @@ static int create_plan(unsigned log_n, unsigned num_buffers, bool reverse, zp_nt
     st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipMalloc(&p->d_work, p->n * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) st = ZP_ERR_HIP;
+  if (st == ZP_OK && hipStreamCreateWithFlags(&p->h2d, hipStreamNonBlocking) != hipSuccess) st = ZP_ERR_HIP;
+  if (st == ZP_OK && hipStreamCreateWithFlags(&p->d2h, hipStreamNonBlocking) != hipSuccess) st = ZP_ERR_HIP;
   if (st == ZP_OK) {
     p->buffers.resize(num_buffers);
   }
@@ -412,16 +423,19 @@ extern "C" int zp_ntt_create_reverse(unsigned log_row_size, unsigned num_buffers
 
 extern "C" int zp_ntt_destroy(zp_ntt_plan *p) {
   if (!p) return ZP_OK;
-  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (hipStream_t st : {p->h2d, p->stream, p->d2h})
+    if (st) (void)hipStreamSynchronize(st);
   for (auto &b : p->buffers) {
     if (b.h_in) (void)hipHostFree(b.h_in);
     if (b.h_out) (void)hipHostFree(b.h_out);
     if (b.d_buf) (void)hipFree(b.d_buf);
-    if (b.done) (void)hipEventDestroy(b.done);
+    for (hipEvent_t e : {b.in_done, b.comp_done, b.done})
+      if (e) (void)hipEventDestroy(e);
   }
   if (p->d_scratch) (void)hipFree(p->d_scratch);
   if (p->d_work) (void)hipFree(p->d_work);
-  if (p->stream) (void)hipStreamDestroy(p->stream);
+  for (hipStream_t st : {p->h2d, p->stream, p->d2h})
+    if (st) (void)hipStreamDestroy(st);
   delete p;
   return ZP_OK;
 }
@@ -502,6 +516,8 @@ extern "C" int zp_ntt_request_buffer(zp_ntt_plan *p, int *id) {
       ZP_HIP(hipHostMalloc(&b.h_in, p->n * 8, hipHostMallocDefault));
       ZP_HIP(hipHostMalloc(&b.h_out, p->n * 8, hipHostMallocDefault));
       ZP_HIP(hipMalloc(&b.d_buf, p->n * 8));
+      ZP_HIP(hipEventCreateWithFlags(&b.in_done, hipEventDisableTiming));
+      ZP_HIP(hipEventCreateWithFlags(&b.comp_done, hipEventDisableTiming));
       ZP_HIP(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
     }
     b.in_use = true;
@@ -524,26 +540,35 @@ extern "C" uint64_t *zp_ntt_buffer_output(zp_ntt_plan *p, int id) {
   return p->buffers[id].h_out;
 }
 
+// driver.cpp:277-284.  A buffer still in flight may be enqueued again
+// (bench_throughput.cpp:76-86 enqueues each buffer many times before
+// waiting): the new copy-in waits for the previous copy-out of the buffer,
+// and every evaluation recomputes the output from the pinned input.
 extern "C" int zp_ntt_enqueue(zp_ntt_plan *p, int id, int dir) {
   ZP_TRY(check_buffer(p, id));
   UserBuffer &b = p->buffers[id];
   ZP_CHECK(b.in_use, ZP_ERR_STATE, "zp_ntt_enqueue: buffer %d not requested", id);
-  ZP_CHECK(!b.enqueued, ZP_ERR_STATE, "zp_ntt_enqueue: buffer %d already enqueued", id);
   ZP_TRY(check_dir(p, dir, "zp_ntt_enqueue"));
-  ZP_HIP(hipMemcpyAsync(b.d_buf, b.h_in, p->n * 8, hipMemcpyHostToDevice, p->stream));
+  if (b.enqueued) ZP_HIP(hipStreamWaitEvent(p->h2d, b.done, 0));
+  ZP_HIP(hipMemcpyAsync(b.d_buf, b.h_in, p->n * 8, hipMemcpyHostToDevice, p->h2d));
+  ZP_HIP(hipEventRecord(b.in_done, p->h2d));
+  ZP_HIP(hipStreamWaitEvent(p->stream, b.in_done, 0));
   ZP_TRY(plan_launch(p, b.d_buf, dir, p->stream));
-  ZP_HIP(hipMemcpyAsync(b.h_out, b.d_buf, p->n * 8, hipMemcpyDeviceToHost, p->stream));
-  ZP_HIP(hipEventRecord(b.done, p->stream));
+  ZP_HIP(hipEventRecord(b.comp_done, p->stream));
+  ZP_HIP(hipStreamWaitEvent(p->d2h, b.comp_done, 0));
+  ZP_HIP(hipMemcpyAsync(b.h_out, b.d_buf, p->n * 8, hipMemcpyDeviceToHost, p->d2h));
+  ZP_HIP(hipEventRecord(b.done, p->d2h));
   b.enqueued = true;
   return ZP_OK;
 }
 
+// driver.cpp:246-249: blocks until the buffer's most recent evaluation has
+// been copied out.  Waiting again returns at once.
 extern "C" int zp_ntt_wait(zp_ntt_plan *p, int id) {
   ZP_TRY(check_buffer(p, id));
   UserBuffer &b = p->buffers[id];
   ZP_CHECK(b.in_use && b.enqueued, ZP_ERR_STATE, "zp_ntt_wait: buffer %d not enqueued", id);
   ZP_HIP(hipEventSynchronize(b.done));
-  b.enqueued = false;
   return ZP_OK;
 }
 

@@ -6,6 +6,11 @@
 //                        reference.cpp:60-84.
 //   zp_ntt_host bench    (bench_latency.cpp)  latency percentiles through the
 //                        buffer API (bench_latency.cpp:101-117).
+//   zp_ntt_host throughput (bench_throughput.cpp) every buffer loaded once,
+//                        then E evaluations enqueued round-robin over the
+//                        buffers without waiting, then E waits; reports
+//                        NTTs/s and checks every buffer's output
+//                        (bench_throughput.cpp:60-107).
 //   zp_ntt_host evaluate (evaluate_given.cpp) hex text in, "0x%016x" lines out
 //                        (evaluate_given.cpp:19-76); --expected-filename adds a
 //                        diff against a file in the same format.
@@ -221,13 +226,47 @@ static int run(const Args &a) {
            lat.back());
     return 0;
   }
+  if (a.mode == "throughput") {
+    const uint64_t nb = drv.max_num_buffers_in_flight();
+    std::vector<vec64> inputs(nb, vec64(n)), want(nb, vec64(n));
+    std::vector<zpgpu::NttGpuDriver::UserBuffer *> bufs;
+    for (uint64_t t = 0; t < nb; t++) {
+      for (uint64_t i = 0; i < n; i++) {
+        uint64_t v;
+        do v = rng(); while (v >= P);
+        inputs[t][i] = v;
+      }
+      bufs.push_back(drv.request_buffer());
+      drv.transfer_input_vector_to_user_buffer(bufs[t], inputs[t].data());
+      // expected output: one blocking evaluation of the same input
+      drv.enqueue_evaluation_async(bufs[t], dir);
+      drv.wait_for_result(bufs[t]);
+      drv.transfer_user_buffer_to_output_vector(bufs[t], want[t].data());
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t e = 0, t = 0; e < a.evals; e++, t = (t + 1) % nb) drv.enqueue_evaluation_async(bufs[t], dir);
+    for (uint64_t e = 0, t = 0; e < a.evals; e++, t = (t + 1) % nb) drv.wait_for_result(bufs[t]);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int bad = 0;
+    for (uint64_t t = 0; t < nb; t++) {
+      drv.transfer_user_buffer_to_output_vector(bufs[t], fwd.data());
+      if (fwd != want[t]) {
+        std::cout << "Incorrect result in buffer " << t << std::endl;
+        bad = 1;
+      }
+      drv.free_buffer(bufs[t]);
+    }
+    printf("Time taken for %llu NTTs: %.6fs\nAmortized time taken per NTT %.6fs\nThroughput: %.3f NTTs/second\n",
+           (unsigned long long)a.evals, secs, secs / (double)a.evals, (double)a.evals / secs);
+    return bad;
+  }
   throw std::runtime_error("unknown mode " + a.mode);
 }
 
 int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr,
-            "usage: %s test|bench|evaluate [--log-n K | --log-row-size R] [--core-type NTT|REVERSE]\n"
+            "usage: %s test|bench|throughput|evaluate [--log-n K | --log-row-size R] [--core-type NTT|REVERSE]\n"
             "       [--num-test-cases T] [--num-evaluations E] [--input-filename F --output-filename F\n"
             "       [--expected-filename F]] [--inverse]\n",
             argv[0]);

@@ -92,7 +92,14 @@ int zp_ntt_simple_evaluate(zp_ntt_plan *plan, uint64_t *out, const uint64_t *in,
 /* Asynchronous buffer API, NttFpgaDriver::request_buffer / transfer_* /
  * enqueue_evaluation_async / wait_for_result / free_buffer
  * (driver.h:146-186, driver.cpp:148-286).  Buffers are ids in
- * [0, max_buffers_in_flight).  Input and output are pinned host vectors. */
+ * [0, max_buffers_in_flight).  Input and output are pinned host vectors.
+ * An evaluation copies the input in, transforms and copies the output back;
+ * copies run on their own streams, so one buffer's transfers overlap the
+ * (serialized) transforms of the others.  A buffer still in flight may be
+ * enqueued again (bench_throughput.cpp:76-86 does so): the evaluations run
+ * in order, each from the buffer's input.  zp_ntt_wait returns once the
+ * buffer's most recent evaluation is copied out; waiting again is a no-op.
+ * zp_ntt_wait on a buffer never enqueued since its request is ZP_ERR_STATE. */
 int zp_ntt_request_buffer(zp_ntt_plan *plan, int *buffer_id); /* ZP_ERR_NO_BUFFER when all busy */
 uint64_t *zp_ntt_buffer_input(zp_ntt_plan *plan, int buffer_id);
 uint64_t *zp_ntt_buffer_output(zp_ntt_plan *plan, int buffer_id);

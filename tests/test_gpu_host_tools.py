@@ -15,7 +15,9 @@ TOOL = os.path.join(ROOT, "hardcaml_zprize_amd", "zp_ntt_host")
 
 @pytest.mark.parametrize("args", [["test", "--log-n", "12", "--num-test-cases", "1"],
                                   ["test", "--log-n", "16", "--num-test-cases", "3"],
-                                  ["bench", "--log-n", "14", "--num-evaluations", "10"]])
+                                  ["bench", "--log-n", "14", "--num-evaluations", "10"],
+                                  ["throughput", "--log-n", "16", "--num-evaluations", "40"],
+                                  ["throughput", "--log-n", "20", "--num-evaluations", "20", "--inverse"]])
 def test_host_tool(args):
     print("host tool", args, flush=True)
     r = subprocess.run([TOOL] + args, capture_output=True, text=True, timeout=120)
@@ -23,6 +25,8 @@ def test_host_tool(args):
     assert r.returncode == 0
     if args[0] == "test":
         assert "FAILED" not in r.stdout and "OK" in r.stdout
+    if args[0] == "throughput":  # bench_throughput.cpp:94-103 output, every buffer checked
+        assert "Incorrect result" not in r.stdout and "NTTs/second" in r.stdout
 
 
 def test_host_tool_rejects_unknown_flag():

@@ -91,6 +91,41 @@ def test_buffer_api_and_errors():
             d.simple_evaluate(np.zeros((1 << 13) + 1, dtype=np.uint64))  # driver.cpp:289-297
 
 
+def test_buffer_reenqueue_round_robin():
+    """bench_throughput.cpp:76-86: each buffer enqueued several times without a
+    wait in between, then one wait per evaluation.  Every evaluation restarts
+    from the buffer's input, so a buffer ends holding the transform of its
+    input in the direction of its last evaluation."""
+    _say("ntt buffer re-enqueue")
+    with zntt.NttGpuDriver(14, num_buffers=4) as d:
+        bufs = [d.request_buffer() for _ in range(4)]
+        with pytest.raises(zntt.ZpError):
+            d.wait_for_result(bufs[0])  # never enqueued since the request
+        inputs = [_rand(1 << 14, 90 + i) for i in range(4)]
+        for b, a in zip(bufs, inputs):
+            d.transfer_input_vector_to_user_buffer(b, a)
+        last = {}
+        for e in range(23):
+            t, dr = e % 4, (e // 4) % 2
+            d.enqueue_evaluation_async(bufs[t], dr)
+            last[t] = dr
+        for e in range(23):
+            d.wait_for_result(bufs[e % 4])
+        for t, b in enumerate(bufs):
+            out = d.transfer_user_buffer_to_output_vector(b)
+            assert np.array_equal(out, oracle.ntt(inputs[t], inverse=bool(last[t]))), t
+            d.free_buffer(b)
+        # a re-requested buffer starts over: waiting before an enqueue is an error again
+        b = d.request_buffer()
+        with pytest.raises(zntt.ZpError):
+            d.wait_for_result(b)
+        d.transfer_input_vector_to_user_buffer(b, inputs[0])
+        d.enqueue_evaluation_async(b, zntt.FORWARD)
+        d.wait_for_result(b)
+        assert np.array_equal(d.transfer_user_buffer_to_output_vector(b), oracle.ntt(inputs[0]))
+        d.free_buffer(b)
+
+
 def test_batch_mixed_sizes():
     _say("ntt batch")
     rng = np.random.default_rng(4)
