@@ -19,11 +19,7 @@ DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotun
 
 def default_window(log_n):
     """The library's built-in policy (msm.hip auto_window)."""
-    if log_n <= 8:
-        return 8
-    if log_n >= 24:
-        return 16
-    return max(8, log_n - 6)
+    return min(20, max(8, log_n - 6))
 
 
 def load(path=DEFAULT_PATH):
@@ -49,7 +45,7 @@ def tune_msm(ctx, d_scalars, log_sizes, candidates=None, reps=3):
     try:
         for lg in log_sizes:
             base = default_window(lg)
-            cands = candidates or sorted({min(16, max(8, base + d)) for d in (-2, -1, 0, 1, 2)})
+            cands = candidates or sorted({min(20, max(8, base + d)) for d in (-2, -1, 0, 1, 2)})
             times = {}
             for w in cands:
                 ws.run([(ctx, 1 << lg, d_scalars, w)])  # warm-up (workspace growth)

@@ -75,12 +75,14 @@ static void layout_finish(HostLayout &L) {
 }
 
 // Window width for n points (the size-class policy; "window=<c>" overrides).
+// c = log2 N - 6 in [8, 20]: measured best or within 7 % at 2^22 (c 14-18:
+// 15.5-16.7 ms), 2^24 (c 18: 45 ms; 16: 49-52, 20: 48) and 2^26 (c 20:
+// 150-154 ms; 16: 168-171, 22: 164-170).  Windows wider than 15 bits take the
+// two-level bucket sort.
 static int auto_window(size_t n) {
   int lg = 0;
   while ((1ull << (lg + 1)) <= n) lg++;
-  if (lg <= 8) return 8;
-  if (lg >= 24) return 16;
-  return std::max(8, lg - 6);
+  return std::min(20, std::max(8, lg - 6));
 }
 
 struct MsmOptions {
@@ -92,6 +94,11 @@ struct MsmOptions {
   bool bad_coords = false;  // coords= neither "canonical" nor "montgomery"
   int log_u = -1;  // entries per work unit, -1 = auto
   int log_g = -1;  // buckets per window group, -1 = auto
+  // Bucket sort: 0 auto, 1 single-level LDS counting sort (windows of at
+  // most 2^15 buckets), 2 two-level (any width), 3 global atomics.
+  int sort = 0;
+  bool bad_sort = false;
+  int sort_lbits = -1;  // two-level: low bucket bits sorted in pass B, -1 = auto
 };
 
 static MsmOptions parse_options(const char *s, size_t len) {
@@ -117,6 +124,11 @@ static MsmOptions parse_options(const char *s, size_t len) {
         o.log_u = atoi(v.c_str());
       } else if (k == "log_g") {
         o.log_g = atoi(v.c_str());
+      } else if (k == "sort") {
+        o.sort = v == "auto" ? 0 : v == "lds" ? 1 : v == "msd" ? 2 : v == "atomic" ? 3 : -1;
+        o.bad_sort = o.sort < 0;
+      } else if (k == "sort_lbits") {
+        o.sort_lbits = atoi(v.c_str());
       }
     }
     pos = end + 1;
@@ -129,6 +141,8 @@ static int validate_options(const MsmOptions &o, const char *who) {
            who, o.window);
   ZP_CHECK(o.log_u <= 12 && o.log_g <= 10, ZP_ERR_INVALID_ARG, "%s: log_u/log_g out of range", who);
   ZP_CHECK(!o.bad_coords, ZP_ERR_INVALID_ARG, "%s: coords= must be canonical or montgomery", who);
+  ZP_CHECK(!o.bad_sort, ZP_ERR_INVALID_ARG, "%s: sort= must be auto, lds, msd or atomic", who);
+  ZP_CHECK(o.sort_lbits <= 12, ZP_ERR_INVALID_ARG, "%s: sort_lbits=%d > 12", who, o.sort_lbits);
   return ZP_OK;
 }
 
@@ -344,8 +358,10 @@ struct zp_msm_workspace {
   size_t cap_K = 0, cap_K2 = 0, cap_K3 = 0, cap_K4 = 0, cap_K5 = 0, cap_Kb = 0;
   size_t cap_entries = 0, cap_units = 0, cap_units2 = 0, cap_groups = 0, cap_gs = 0, cap_win = 0, cap_wsum = 0;
   size_t cap_jobs = 0, cap_pt0 = 0, cap_b0 = 0, cap_scan = 0;
-  size_t cap_digits = 0, cap_counts = 0, cap_scanned = 0, cap_segs = 0;
+  size_t cap_digits = 0, cap_counts = 0, cap_scanned = 0, cap_segs = 0, cap_tmp = 0, cap_wg0 = 0;
   int32_t *digits = nullptr;
+  uint2 *tmp = nullptr;         // two-level sort: pass A output
+  uint32_t *win_g0 = nullptr;   // two-level sort: first group of each window
   uint32_t *counts = nullptr, *scanned = nullptr;
   SegDev *segs = nullptr;
   uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
@@ -361,7 +377,7 @@ struct zp_msm_workspace {
 
   void release() {
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
-                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums};
+                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
@@ -576,17 +592,48 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
     d.chunks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, h.n >> 16));
     d.chunk_len = (uint32_t)std::max<uint64_t>(1, (h.n + d.chunks - 1) / d.chunks);
     d.dig0 = entries;
-    d.cnt0 = counters;
-    counters += (K - d.bucket0) * d.chunks;
     for (int w = 0; w < L.W; w++)
       for (uint32_t c = 0; c < d.chunks; c++) segs.push_back(SegDev{(uint32_t)j, d.win0 + (uint32_t)w, c});
     points += h.n;
     entries += (uint64_t)h.n * L.W;
     n_max = std::max<uint64_t>(n_max, h.n);
   }
-  // The LDS histogram holds one window's buckets (<= 2^15 u32 = 128 KiB).
+  // Sort: the single-level LDS histogram holds one window's buckets (<= 2^15
+  // u32 = 128 KiB); the two-level sort takes any width (pass A: <= 2^12
+  // groups per window in LDS, pass B: 2^lbits buckets per group).
   const char *force_atomic = getenv("ZP_MSM_ATOMIC_SORT");
-  const bool sorted_path = max_nb <= 32768 && !(force_atomic && atoi(force_atomic));
+  int sort = bopts.sort ? bopts.sort : jobs[0].ctx->opts.sort;
+  if (force_atomic && atoi(force_atomic)) sort = 3;
+  if (sort == 0) sort = max_nb <= 32768 ? 1 : 2;
+  if (sort == 1 && max_nb > 32768) sort = 2;
+  const bool sorted_path = sort == 1, msd_path = sort == 2;
+  // Two-level split per window: group = bucket >> lbits with about 2^11
+  // groups per window (measured at 2^26, window 20: 2^11 groups 157 ms, 2^9
+  // 190 ms, 2^7 302 ms: pass B is one block per group), and at most 2^12
+  // (pass A's LDS histogram); a few-bucket top window gets one bucket per group.
+  const int lb_opt = bopts.sort_lbits >= 0 ? bopts.sort_lbits : jobs[0].ctx->opts.sort_lbits;
+  uint64_t GT = 0;
+  uint32_t max_lbits = 0, max_groups = 1;
+  for (size_t j = 0; j < count; j++) {
+    MsmJobDev &d = jd[j];
+    d.cnt0 = counters;
+    d.g0 = (uint32_t)GT;
+    uint64_t kj = 0;
+    for (uint32_t w = d.win0; w < d.win0 + d.W; w++) {
+      WinDev &wd = wins[w];
+      uint32_t log_nb = 0;
+      while ((1u << log_nb) < wd.nb) log_nb++;
+      const uint32_t lo = log_nb > 12 ? log_nb - 12 : 0;
+      wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > 11 ? log_nb - 11 : 0));
+      wd.g0 = (uint32_t)GT;
+      const uint32_t H = bucket_groups(wd.nb, wd.lbits);
+      GT += H;
+      max_groups = std::max(max_groups, H);
+      max_lbits = std::max(max_lbits, wd.lbits);
+      kj += wd.nb;
+    }
+    counters += (msd_path ? GT - d.g0 : kj) * d.chunks;
+  }
   ZP_CHECK(points < (1ull << 31) && K < (1ull << 31) && entries < (1ull << 32), ZP_ERR_TOO_LARGE,
            "MSM batch too large (%llu points, %llu buckets)", (unsigned long long)points, (unsigned long long)K);
   const uint32_t Wt = (uint32_t)wins.size();
@@ -620,7 +667,11 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   size_t scan_bytes = 0, scan_bytes2 = 0;
   ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
                                  stream));
-  if (sorted_path) {
+  if (msd_path) {
+    ZP_TRY(grow(ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1)));
+    ZP_TRY(grow(ws.win_g0, ws.cap_wg0, std::max<size_t>(wins.size(), 1)));
+  }
+  if (sorted_path || msd_path) {
     ZP_TRY(grow(ws.digits, ws.cap_digits, std::max<uint64_t>(entries, 1)));
     ZP_TRY(grow(ws.counts, ws.cap_counts, counters + 1));
     ZP_TRY(grow(ws.scanned, ws.cap_scanned, counters + 1));
@@ -634,8 +685,12 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(hipMemcpyAsync(ws.pt0, pt0.data(), count * 4, hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.b0, b0.data(), count * 4, hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.gstart, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice, stream));
-  if (sorted_path && !segs.empty())
+  if ((sorted_path || msd_path) && !segs.empty())
     ZP_HIP(hipMemcpyAsync(ws.segs, segs.data(), segs.size() * sizeof(SegDev), hipMemcpyHostToDevice, stream));
+  std::vector<uint32_t> win_g0(wins.size());
+  for (size_t w = 0; w < wins.size(); w++) win_g0[w] = wins[w].g0;
+  if (msd_path && !wins.empty())
+    ZP_HIP(hipMemcpyAsync(ws.win_g0, win_g0.data(), wins.size() * 4, hipMemcpyHostToDevice, stream));
   BatchDev B;
   B.jobs = ws.jobs;
   B.job_pt0 = ws.pt0;
@@ -673,6 +728,28 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
                          B, ws.segs, ws.digits, ws.scanned, ws.vals);
       ZP_HIP(hipGetLastError());
     }
+  } else if (msd_path) {
+    const size_t lds_a = (size_t)max_groups * sizeof(uint32_t);
+    const size_t lds_b = ((2u << max_lbits) + 1024) * sizeof(uint32_t);
+    if (points) {
+      hipLaunchKernelGGL(k_digits_make, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.digits);
+      ZP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_msd_count, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs, ws.digits,
+                       ws.counts);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, stream, ws.counts + counters, (uint64_t)1, 0u);
+    ZP_HIP(hipGetLastError());
+    size_t tb0 = ws.cap_scan;
+    ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb0, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
+                                   rocprim::plus<uint32_t>(), stream));
+    if (points) {
+      hipLaunchKernelGGL(k_msd_scatter, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
+                         ws.digits, ws.scanned, ws.tmp, ws.vals);
+      ZP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_msd_group, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt, (uint32_t)GT,
+                       max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
+    ZP_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);
     ZP_HIP(hipGetLastError());

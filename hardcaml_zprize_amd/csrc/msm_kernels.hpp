@@ -43,6 +43,8 @@ struct MsmJobDev {
   uint32_t chunk_len;       // points per chunk
   uint64_t dig0;            // first digit slot: digit(w, i) at dig0 + w * n + i
   uint64_t cnt0;            // first per-(bucket, chunk) counter: cnt0 + b_local * C + c
+                            // (two-level path: per-(group, chunk), cnt0 + (g - g0) * C + c)
+  uint32_t g0;              // two-level path: global index of the job's first bucket group
 };
 
 // One block of the sorted path: window w (global) of job j, chunk c.
@@ -55,6 +57,8 @@ struct WinDev {
   uint32_t off, width;  // bit offset and width of the window
   uint32_t nb, base;    // buckets, first global bucket
   uint32_t top;         // 1 = unsigned top window (slice + carry)
+  uint32_t g0;          // two-level path: global index of the window's first bucket group
+  uint32_t lbits;       // two-level path: group = bucket >> lbits
 };
 
 // Largest i in [0, n) with key[i] <= v (keys ascending, key[0] == 0).
@@ -208,6 +212,150 @@ __global__ void __launch_bounds__(1024) k_seg_scatter(BatchDev B, const SegDev *
     if (d == 0) continue;
     const uint32_t pos = atomicAdd(&cursor[(uint32_t)(d < 0 ? -d : d) - 1], 1u);
     vals[pos] = i | (d < 0 ? 0x80000000u : 0u);
+  }
+}
+
+// ---- two-level sorted path (windows of any width up to 2^21 buckets) ----
+// Pass A partitions each window's entries by the high part of the bucket,
+// group h = b >> lbits (lbits per window), with per-(window, chunk) LDS
+// histograms as above; pass B sorts each group by the low lbits in a single
+// block and writes the
+// final bucket offsets.  Every write then lands in a run of consecutive
+// slots -- a chunk's entries of one group, a group's entries of one bucket --
+// where the single-level scatter writes about two entries per bucket per
+// chunk.  Group sizes are data dependent, so pass B finds its range in the
+// scanned pass-A counters on the device: group g's entries are
+// [scanned[i], scanned[i + C]) with i its first counter.
+
+__host__ __device__ __forceinline__ uint32_t bucket_groups(uint32_t nb, uint32_t lbits) { return ((nb - 1) >> lbits) + 1; }
+
+// *ctr += 1 for each active lane; returns the value before this lane's
+// increment.  When every active lane of the wave has the same counter, one
+// lane adds the wave's count (hot buckets would otherwise serialize the LDS
+// atomics 64 ways).
+__device__ __forceinline__ uint32_t lds_claim(uint32_t *base, uint32_t key) {
+  const uint64_t active = __ballot(1);
+  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+  if (__ballot(key == k0) == active) {
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(active >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)active, 0u));
+    uint32_t first = 0;
+    if (rank == 0) first = atomicAdd(&base[k0], (uint32_t)__popcll(active));
+    // the leader is the lowest active lane
+    first = __shfl(first, __ffsll((long long)active) - 1);
+    return first + rank;
+  }
+  return atomicAdd(&base[key], 1u);
+}
+
+// Pass A count: per segment (job, window, chunk), the chunk's nonzero digits
+// by group, to counts[cnt0 + (g - job g0) * C + c] (every group, zeros included).
+__global__ void __launch_bounds__(1024) k_msd_count(BatchDev B, const SegDev *segs, const int32_t *digits,
+                                                    uint32_t *counts) {
+  extern __shared__ uint32_t hist[];
+  const SegDev sg = segs[blockIdx.x];
+  const MsmJobDev jb = B.jobs[sg.job];
+  const WinDev wd = B.wins[sg.win];
+  const uint32_t lbits = wd.lbits;
+  const uint32_t H = bucket_groups(wd.nb, lbits);
+  for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) hist[h] = 0;
+  __syncthreads();
+  const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
+  const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int32_t d = dig[i];
+    if (d != 0) lds_claim(hist, ((uint32_t)(d < 0 ? -d : d) - 1) >> lbits);
+  }
+  __syncthreads();
+  uint32_t *dst = counts + jb.cnt0 + (uint64_t)(wd.g0 - jb.g0) * jb.chunks + sg.chunk;
+  for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) dst[(uint64_t)h * jb.chunks] = hist[h];
+}
+
+// Pass A scatter: tmp[pos] = (point | sign << 31, local bucket), grouped by
+// (window, group), chunks in order within a group.  A window whose groups are
+// single buckets (lbits = 0) is sorted by now: its entries go straight to
+// vals (same slots), and pass B only writes their offsets.
+__global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *segs, const int32_t *digits,
+                                                      const uint32_t *scanned, uint2 *tmp, uint32_t *vals) {
+  extern __shared__ uint32_t cursor[];
+  const SegDev sg = segs[blockIdx.x];
+  const MsmJobDev jb = B.jobs[sg.job];
+  const WinDev wd = B.wins[sg.win];
+  const uint32_t lbits = wd.lbits;
+  const uint32_t H = bucket_groups(wd.nb, lbits);
+  const uint32_t *src = scanned + jb.cnt0 + (uint64_t)(wd.g0 - jb.g0) * jb.chunks + sg.chunk;
+  for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) cursor[h] = src[(uint64_t)h * jb.chunks];
+  __syncthreads();
+  const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
+  const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int32_t d = dig[i];
+    if (d == 0) continue;
+    const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
+    const uint32_t pos = lds_claim(cursor, b >> lbits);
+    const uint32_t v = i | (d < 0 ? 0x80000000u : 0u);
+    if (lbits == 0)
+      vals[pos] = v;
+    else
+      tmp[pos] = make_uint2(v, b);
+  }
+}
+
+// Pass B: block g sorts bucket group g (window w, high part h) by the low
+// lbits of its window: histogram, exclusive scan, off/cnt of the group's
+// buckets, then placement into vals.  The last group also writes off[K] (the
+// total).  LDS: 2 * 2^max_lbits + 1024 words.
+__global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *win_g0, uint32_t Wt, uint32_t GT,
+                                                    uint32_t max_lbits, const uint32_t *scanned, const uint2 *tmp,
+                                                    uint32_t *vals, uint32_t *off, uint32_t *cnt) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t g = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t w = upper_index(win_g0, Wt, g);
+  const WinDev wd = B.wins[w];
+  const MsmJobDev jb = B.jobs[wd.job];
+  const uint64_t ci = jb.cnt0 + (uint64_t)(g - jb.g0) * jb.chunks;
+  const uint32_t start = scanned[ci], end = scanned[ci + jb.chunks];
+  const uint32_t b0 = (g - wd.g0) << wd.lbits;
+  const uint32_t nl = min(1u << wd.lbits, wd.nb - b0);  // buckets in this group
+  uint32_t *hist = lds, *cur = lds + (1u << max_lbits), *part = cur + (1u << max_lbits);
+  if (wd.lbits == 0) {  // one bucket, placed by pass A
+    if (tid == 0) {
+      off[wd.base + b0] = start;
+      cnt[wd.base + b0] = end - start;
+      if (g == GT - 1) off[B.K] = end;
+    }
+    return;
+  }
+  for (uint32_t l = tid; l < nl; l += nt) hist[l] = 0;
+  __syncthreads();
+  for (uint32_t p = start + tid; p < end; p += nt) lds_claim(hist, tmp[p].y - b0);
+  __syncthreads();
+  // exclusive scan of hist[0, nl): each thread sums a run of `per`, a
+  // Hillis-Steele scan over the 1024 run sums, then each run is expanded.
+  const uint32_t per = (nl + nt - 1) / nt;
+  const uint32_t l0 = min(nl, tid * per), l1 = min(nl, l0 + per);
+  uint32_t sum = 0;
+  for (uint32_t l = l0; l < l1; l++) sum += hist[l];
+  part[tid] = sum;
+  __syncthreads();
+  for (uint32_t st = 1; st < nt; st <<= 1) {
+    const uint32_t v = part[tid] + (tid >= st ? part[tid - st] : 0u);
+    __syncthreads();
+    part[tid] = v;
+    __syncthreads();
+  }
+  uint32_t run = start + part[tid] - sum;  // exclusive prefix of this thread's run
+  for (uint32_t l = l0; l < l1; l++) {
+    const uint32_t c = hist[l];
+    off[wd.base + b0 + l] = run;
+    cnt[wd.base + b0 + l] = c;
+    cur[l] = run;
+    run += c;
+  }
+  if (g == GT - 1 && tid == 0) off[B.K] = end;
+  __syncthreads();
+  for (uint32_t p = start + tid; p < end; p += nt) {
+    const uint2 e = tmp[p];
+    vals[lds_claim(cur, e.y - b0)] = e.x;
   }
 }
 

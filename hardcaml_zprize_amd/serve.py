@@ -53,8 +53,12 @@ class Job:
         n = 1 << self.log_n
         if self.kind == NTT:
             return 16 * n  # vector + 4-step scratch
-        # scalars + sorted entries (<= 32 windows) + work units + buckets, generous
-        return 32 * n + 4 * 32 * n + 64 * n + (1 << 26)
+        # scalars; per (point, window) a digit, a pass-A entry and a sorted
+        # entry (16 B); bucket sums, work-unit partials and window groups
+        # (192 B points, generous); window c = min(20, max(8, log n - 6))
+        c = min(20, max(8, self.log_n - 6))
+        w = -(-253 // c)
+        return 32 * n + 16 * w * n + 4 * 192 * w * (1 << (c - 1)) + (1 << 26)
 
     def wait(self, timeout=None):
         self._event.wait(timeout)

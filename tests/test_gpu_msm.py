@@ -249,10 +249,13 @@ def test_batch_mixed_jobs_two_tables():
         expect.append(oracle.g1_scalar_mul(g, e))
     ws = zmsm.MsmWorkspace()
     try:
-        for rep in range(2):
-            out = ws.run(jobs)
+        # the same jobs through each bucket sort (the two-level one also at a
+        # window no single-level LDS histogram holds)
+        for rep, opts in enumerate(["", "", "sort=msd", "sort=msd,window=18", "sort=atomic"]):
+            out = ws.run(jobs, opts)
             for j, (o, e) in enumerate(zip(out, expect)):
-                assert oracle.affine_equal(oracle.projective_to_affine(o), e), (rep, j)
+                assert oracle.affine_equal(oracle.projective_to_affine(o), e), (rep, opts, j)
+        out = ws.run(jobs)
         alone = [ctx.mult_prefix_device(n, p) for ctx, n, p in jobs]
         assert all(a.tobytes() == b.tobytes() for a, b in zip(alone, out))
     finally:
@@ -325,3 +328,76 @@ def test_hot_buckets_closed_form():
             e = oracle.dot_mod_r(sc, ks)
             assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(g, e)), name
             assert dt < 5.0, (name, dt)
+
+
+@pytest.mark.parametrize("window", [8, 12, 15, 16, 17, 18, 19, 20, 21])
+def test_two_level_sort_windows(window):
+    """The two-level bucket sort (sort=msd) at every window width, including
+    widths whose top window holds 1-6 bits (18, 19: almost every point lands
+    in its few buckets): closed form ((sum s_i k_i) mod r) G, and the same
+    point as the single-level sort where that one applies."""
+    _say(f"msm two-level sort window={window}")
+    rng = random.Random(600 + window)
+    n = 5000
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    s = [rng.randrange(R) for _ in range(n)]
+    sc = _words(s, 4)
+    exp = oracle.g1_scalar_mul(oracle.generator(), sum(a * b for a, b in zip(s, ks)) % R)
+    outs = {}
+    for sort in ("msd", "atomic") + (("lds",) if window <= 16 else ()):
+        with zmsm.MsmContext(pts, f"window={window},sort={sort}") as ctx:
+            outs[sort] = ctx.mult(sc, 1)
+        assert oracle.affine_equal(oracle.projective_to_affine(outs[sort][0]), exp), sort
+
+
+@pytest.mark.parametrize("lbits", [0, 3, 7, 12])
+def test_two_level_sort_split(lbits):
+    """Pass B sorting 2^lbits buckets per group, from one bucket per group
+    (pass A does the whole sort) to more buckets than a window has."""
+    _say(f"msm two-level sort lbits={lbits}")
+    rng = random.Random(700 + lbits)
+    n = 3000
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    s = [rng.randrange(R) for _ in range(n)]
+    with zmsm.MsmContext(pts, f"window=11,sort=msd,sort_lbits={lbits}") as ctx:
+        out = ctx.mult(_words(s, 4), 1)
+    exp = oracle.g1_scalar_mul(oracle.generator(), sum(a * b for a, b in zip(s, ks)) % R)
+    assert oracle.affine_equal(oracle.projective_to_affine(out[0]), exp)
+
+
+def test_two_level_sort_hot_buckets():
+    """Hot buckets through the two-level sort at a wide window: every point of
+    a window in one group and one bucket (all equal), a bucket per window at
+    value 1 (all ones), and half the scalars zero."""
+    import time
+
+    rng = random.Random(405)
+    n = 1 << 18
+    ks_int = [rng.randrange(1, R) for _ in range(n)]
+    ks = _words(ks_int, 4)
+    g = oracle.generator()
+    s_eq = rng.randrange(1, R)
+    cases = {
+        "all_equal": np.tile(_words([s_eq], 4), (n, 1)),
+        "all_ones": np.tile(np.array([[1, 0, 0, 0]], dtype=np.uint64), (n, 1)),
+        "half_zero": np.concatenate([np.zeros((n // 2, 4), dtype=np.uint64),
+                                     np.tile(_words([R - 1], 4), (n - n // 2, 1))]),
+    }
+    with zmsm.MsmContext(zmsm.generate_points(ks), "window=20,sort=msd") as ctx:
+        for name, sc in cases.items():
+            _say(f"msm two-level hot buckets {name}")
+            t0 = time.perf_counter()
+            out = ctx.mult(sc, 1)
+            dt = time.perf_counter() - t0
+            e = oracle.dot_mod_r(sc, ks)
+            assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(g, e)), name
+            assert dt < 5.0, (name, dt)
+
+
+def test_sort_option_rejected():
+    from hardcaml_zprize_amd._lib import ZpError
+
+    with pytest.raises(ZpError, match="sort"):
+        zmsm.MsmContext(zmsm.generate_points(_words([1, 2, 3], 4)), "sort=radix")
