@@ -204,11 +204,14 @@ def test_device_scalars_and_prefix():
     ds.free()
 
 
-def test_large_closed_form():
-    """2^20 points generated on the GPU; result vs (sum s_i k_i mod r) G."""
-    _say("msm large 2^20")
-    rng = np.random.default_rng(2024)
-    n = 1 << 20
+@pytest.mark.parametrize("log_n,options", [(20, ""), (24, ""), (24, "window=20"), (23, "window=19")])
+def test_large_closed_form(log_n, options):
+    """Points generated on the GPU; result vs (sum s_i k_i mod r) G.  2^20 at
+    the automatic window takes the single-level sort, 2^24 (window 18) and
+    the wider windows the two-level sort (window 19: a 6-bit top window)."""
+    _say(f"msm large 2^{log_n} {options}")
+    rng = np.random.default_rng(2024 + log_n)
+    n = 1 << log_n
     ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
     ks[:, 3] &= np.uint64((1 << 60) - 1)  # < 2^252 < r
     sc = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
@@ -216,11 +219,9 @@ def test_large_closed_form():
     dk = zmsm.DeviceBuffer(32 * n).upload(ks)
     dp = zmsm.DeviceBuffer(104 * n)
     zmsm.generate_points_device(dk.ptr, dp.ptr, n)
-    with zmsm.MsmContext((dp.ptr, n)) as ctx:
+    with zmsm.MsmContext((dp.ptr, n), options) as ctx:
         out = ctx.mult(sc, 1)
-    kk = [oracle.words_to_int(k) for k in ks]
-    ss = [oracle.words_to_int(s) for s in sc]
-    e = sum(a * b for a, b in zip(ss, kk)) % R
+    e = oracle.dot_mod_r(sc, ks)
     assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(oracle.generator(), e))
     dk.free()
     dp.free()
