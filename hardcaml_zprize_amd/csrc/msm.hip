@@ -395,6 +395,89 @@ struct NcGroup {
   std::vector<uint32_t> idx;  // table indices, ascending
 };
 
+// Host (pageable) scalars -> device through pinned staging.  The caller's
+// array is split into kLanes contiguous slices; each slice is copied by its
+// own host thread into one of that lane's two pinned slabs, and the slab is
+// sent by DMA on the lane's stream while the thread fills the other slab.
+// A plain pageable hipMemcpy runs at ~25 GB/s here; this keeps the PCIe link
+// busy from several DMA queues.  upload() returns once the data is on the
+// device.  One upload at a time per uploader.
+struct HostUploader {
+  static constexpr int kLanes = 8;
+  static constexpr size_t kSlab = size_t(16) << 20;
+  struct Lane {
+    void *slab[2] = {nullptr, nullptr};
+    hipEvent_t sent[2] = {nullptr, nullptr};
+    hipStream_t s = nullptr;
+  };
+  Lane lanes[kLanes];
+  bool ready = false;
+
+  int init() {
+    if (ready) return ZP_OK;
+    for (Lane &l : lanes) {
+      for (int k = 0; k < 2; k++) {
+        ZP_HIP(hipHostMalloc(&l.slab[k], kSlab, hipHostMallocDefault));
+        ZP_HIP(hipEventCreateWithFlags(&l.sent[k], hipEventDisableTiming));
+      }
+      ZP_HIP(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
+    }
+    ready = true;
+    return ZP_OK;
+  }
+
+  void release() {
+    for (Lane &l : lanes) {
+      if (l.s) (void)hipStreamSynchronize(l.s);
+      for (int k = 0; k < 2; k++) {
+        if (l.slab[k]) (void)hipHostFree(l.slab[k]);
+        if (l.sent[k]) (void)hipEventDestroy(l.sent[k]);
+      }
+      if (l.s) (void)hipStreamDestroy(l.s);
+      l = Lane();
+    }
+    ready = false;
+  }
+
+  int upload(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return ZP_OK;
+    if (bytes < 4 * kSlab) {  // small: one pageable copy
+      ZP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+      return ZP_OK;
+    }
+    ZP_TRY(init());
+    int dev = 0;
+    ZP_HIP(hipGetDevice(&dev));
+    const size_t per = ((bytes + kLanes - 1) / kLanes + 4095) & ~size_t(4095);
+    std::atomic<int> failed{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < kLanes; t++) {
+      const size_t b0 = std::min(bytes, per * t), b1 = std::min(bytes, b0 + per);
+      if (b0 >= b1) break;
+      th.emplace_back([&, t, b0, b1] {
+        Lane &l = lanes[t];
+        bool ok = hipSetDevice(dev) == hipSuccess;
+        bool used[2] = {false, false};
+        int k = 0;
+        for (size_t o = b0; ok && o < b1; o += kSlab, k ^= 1) {
+          const size_t len = std::min(kSlab, b1 - o);
+          if (used[k]) ok = hipEventSynchronize(l.sent[k]) == hipSuccess;  // slab k's last DMA done
+          if (!ok) break;
+          memcpy(l.slab[k], (const char *)src + o, len);
+          ok = hipMemcpyAsync((char *)dst + o, l.slab[k], len, hipMemcpyHostToDevice, l.s) == hipSuccess &&
+               hipEventRecord(l.sent[k], l.s) == hipSuccess;
+          used[k] = true;
+        }
+        if (hipStreamSynchronize(l.s) != hipSuccess) ok = false;
+        if (!ok) failed = 1;
+      });
+    }
+    for (auto &x : th) x.join();
+    ZP_CHECK(!failed, ZP_ERR_HIP, "scalar upload through pinned staging failed");
+    return ZP_OK;
+  }
+};
+
 struct zp_msm_context {
   size_t n = 0;
   MsmOptions opts;
@@ -410,7 +493,7 @@ struct zp_msm_context {
   size_t cap_scalars = 0;
   uint64_t *d_scalars2 = nullptr;  // second buffer: the next batch uploads while one computes
   size_t cap_scalars2 = 0;
-  hipStream_t copy_stream = nullptr;
+  HostUploader uploader;  // host scalars of msm_mult
 };
 
 namespace zp {
@@ -1053,7 +1136,7 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   ctx->ws.release();
   if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
   if (ctx->d_scalars2) (void)hipFree(ctx->d_scalars2);
-  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
+  ctx->uploader.release();
   if (ctx->d_pts) (void)hipFree(ctx->d_pts);
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
   if (ctx->d_nc_idx) (void)hipFree(ctx->d_nc_idx);
@@ -1068,16 +1151,17 @@ extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t n
            ZP_ERR_INVALID_ARG, "zp_msm_mult: NULL argument");
   const size_t n = ctx->n;
   ZP_TRY(grow(ctx->d_scalars, ctx->cap_scalars, std::max<size_t>(4 * n, 4)));
-  if (num_batches > 1 && n) {
-    ZP_TRY(grow(ctx->d_scalars2, ctx->cap_scalars2, std::max<size_t>(4 * n, 4)));
-    if (!ctx->copy_stream) ZP_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  if (num_batches > 1 && n) ZP_TRY(grow(ctx->d_scalars2, ctx->cap_scalars2, std::max<size_t>(4 * n, 4)));
+  // The previous call's work on ctx->stream may still read d_scalars.
+  if (n && num_batches) {
+    ZP_HIP(hipStreamSynchronize(ctx->stream));
+    ZP_TRY(ctx->uploader.upload(ctx->d_scalars, scalars, n * 32));
   }
-  if (n && num_batches) ZP_HIP(hipMemcpyAsync(ctx->d_scalars, scalars, n * 32, hipMemcpyHostToDevice, ctx->stream));
   // Batches alternate between two device buffers: while batch b computes on
-  // ctx->stream, a helper thread uploads batch b + 1 on the copy stream
-  // (driver.cpp:495-600 overlaps transfers with work across batches the same
-  // way).  run_batch returns only when batch b is finished, so the buffer the
-  // next upload overwrites is free.
+  // ctx->stream, a helper thread uploads batch b + 1 through the pinned
+  // staging (driver.cpp:495-600 overlaps transfers with work across batches
+  // the same way).  run_batch returns only when batch b is finished, so the
+  // buffer the next upload overwrites is free.
   for (uint64_t b = 0; b < num_batches; b++) {
     uint64_t *cur = (b & 1) ? ctx->d_scalars2 : ctx->d_scalars;
     uint64_t *nxt = (b & 1) ? ctx->d_scalars : ctx->d_scalars2;
@@ -1085,10 +1169,10 @@ extern "C" int zp_msm_mult(zp_msm_context *ctx, g1_projective_t *out, uint64_t n
     std::thread copier;
     if (b + 1 < num_batches && n) {
       const biginteger256_t *hs = scalars + (b + 1) * n;
-      copier = std::thread([&, hs, nxt] {
-        if (hipMemcpyAsync(nxt, hs, n * 32, hipMemcpyHostToDevice, ctx->copy_stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->copy_stream) != hipSuccess)
-          copy_rc = ZP_ERR_HIP;
+      int dev = 0;
+      ZP_HIP(hipGetDevice(&dev));
+      copier = std::thread([&, hs, nxt, dev] {
+        if (hipSetDevice(dev) != hipSuccess || ctx->uploader.upload(nxt, hs, n * 32) != ZP_OK) copy_rc = ZP_ERR_HIP;
       });
     }
     HostJob job{ctx, n, cur, out + b};

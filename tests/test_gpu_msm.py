@@ -402,3 +402,34 @@ def test_sort_option_rejected():
 
     with pytest.raises(ZpError, match="sort"):
         zmsm.MsmContext(zmsm.generate_points(_words([1, 2, 3], 4)), "sort=radix")
+
+
+def test_host_scalars_pinned_upload_batches():
+    """msm_mult with host scalars large enough (64 MiB per batch) for the
+    pinned multi-lane upload, over three batches (each next batch uploads
+    while the previous computes), from a misaligned host array: every batch
+    equals ((sum s_i k_i) mod r) G."""
+    _say("msm pinned upload 3 x 2^21")
+    rng = np.random.default_rng(77)
+    n = 1 << 21
+    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
+    ks[:, 3] &= np.uint64((1 << 60) - 1)
+    raw = np.zeros(3 * n * 32 + 8, dtype=np.uint8)
+    sc = raw[8:].view(np.uint64).reshape(3 * n, 4)  # 8-byte offset from the allocation
+    sc[:] = rng.integers(0, 2**64, size=(3 * n, 4), dtype=np.uint64)
+    sc[:, 3] &= np.uint64((1 << 60) - 1)
+    dk = zmsm.DeviceBuffer(32 * n).upload(ks)
+    dp = zmsm.DeviceBuffer(104 * n)
+    zmsm.generate_points_device(dk.ptr, dp.ptr, n)
+    try:
+        with zmsm.MsmContext((dp.ptr, n)) as ctx:
+            out = ctx.mult(sc, 3)
+            again = ctx.mult(sc[n:2 * n], 1)
+        g = oracle.generator()
+        for b in range(3):
+            e = oracle.dot_mod_r(sc[b * n:(b + 1) * n], ks)
+            assert oracle.affine_equal(oracle.projective_to_affine(out[b]), oracle.g1_scalar_mul(g, e)), b
+        assert again[0].tobytes() == out[1].tobytes()
+    finally:
+        dk.free()
+        dp.free()
