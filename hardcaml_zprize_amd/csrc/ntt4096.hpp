@@ -194,6 +194,25 @@ __device__ __forceinline__ void inner_twiddle(uint64_t (&x)[64], const uint64_t 
   });
 }
 
+// Same product with the loads one group ahead: group g + 1 is in flight while
+// group g multiplies, and w0 holds group 0 (w0[0] unused), which the caller
+// loads early (before the first DFT).  Used where the 16 extra VGPRs fit
+// without spills (the columns kernel: -2.4 % kernel time; the rows kernel
+// spills with it).
+__device__ __forceinline__ void inner_twiddle_pf(uint64_t (&x)[64], const uint64_t *tw, const uint64_t (&w0)[8]) {
+  uint64_t w[2][8];
+  static_for<8>([&](auto i) { w[0][i] = w0[i]; });
+  static_for<8>([&](auto g) {
+    if constexpr (g + 1 < 8) {
+      static_for<8>([&](auto i) { w[(g + 1) & 1][i] = tw[((g + 1) * 8 + i) * 64]; });
+    }
+    static_for<8>([&](auto i) {
+      if constexpr (g * 8 + i > 0) x[g * 8 + i] = gl_mul_lz(x[g * 8 + i], w[g & 1][i]);
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
 struct Ntt4096Args {
   const uint64_t *in;
   uint64_t *out;
@@ -273,23 +292,30 @@ k_ntt4096_cols(Ntt4096Args args) {
   const uint64_t n2 = args.n2;
   const uint32_t lane = a * (uint32_t)n2 + s;  // element (a, s) of the tile's first row block
   uint64_t x[64];
+  // Table loads that do not depend on the data go first: the 4-step twiddle
+  // factors of the epilogue and the first group of inner twiddles.
+  const uint64_t n = args.n1 * n2;
+  const uint64_t e0 = (col * a) & (n - 1);
+  const uint64_t e1 = (col * 64) & (n - 1);
+  const uint64_t th0 = args.tw_hi[e0 >> 12], tl0 = args.tw_lo[e0 & 4095];
+  const uint64_t th1 = args.tw_hi[e1 >> 12], tl1 = args.tw_lo[e1 & 4095];
+  const uint64_t *tw = args.tw4096 + a;
+  uint64_t w0[8];
+  static_for<8>([&](auto i) { w0[i] = i > 0 ? tw[i * 64] : 1; });
   // All 64 loads are issued before any arithmetic (memory-level parallelism
   // for the strided row segments); the waits then come class by class.
   load_order([&](auto m) { x[m] = ld_u64(args.in + (uint64_t)(64 * m) * n2 + (uint64_t)tile * T, lane); });
   __builtin_amdgcn_sched_barrier(0);
   dft64_regs<DIR, true>(x);  // inputs >= p count as their residue
-  inner_twiddle(x, args.tw4096 + a);
+  inner_twiddle_pf(x, tw, w0);
   exchange<T, false>(x, lds, s, a, s, a);
   dft64_regs<DIR, true>(x);
   // x[k''] = output k1 = a + 64 k'' (a now plays k').  4-step twiddle
   // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.  The scratch matrix
   // holds lazy (not necessarily canonical) values; pass 2 canonicalizes the
   // ones its first butterfly stage needs.
-  const uint64_t n = args.n1 * n2;
-  const uint64_t e0 = (col * a) & (n - 1);
-  const uint64_t e1 = (col * 64) & (n - 1);
-  uint64_t w = gl_mul(args.tw_hi[e0 >> 12], args.tw_lo[e0 & 4095]);
-  const uint64_t step = gl_mul(args.tw_hi[e1 >> 12], args.tw_lo[e1 & 4095]);
+  uint64_t w = gl_mul(th0, tl0);
+  const uint64_t step = gl_mul(th1, tl1);
 #pragma unroll
   for (int k = 0; k < 64; k++) {
     st_u64(args.out + (uint64_t)(64 * k) * n2 + (uint64_t)tile * T, lane, gl_mul_lz(x[k], w));
