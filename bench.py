@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=120)
     ap.add_argument("--serve-load", type=float, default=6.0)
     ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-batch-ms", type=float, default=100.0,
+                    help="estimated work per served batch (ms); 0 = every waiting job that fits the budget")
+    ap.add_argument("--serve-load2", type=float, default=0.7,
+                    help="second offered load (x plain capacity) for the latency comparison; 0 = skip")
     ap.add_argument("--skip-msm", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
@@ -384,19 +388,35 @@ def bench_served(args):
     plain = serve.GpuServer(tables, hbm_budget=200e9, workers=1, batching=False)
     serve.run_stream(plain, jobs()[:8], [0.0] * 8)  # warm-up
     cal_s, _ = serve.run_stream(plain, jobs(), [0.0] * njobs)
-    rate = args.serve_load * njobs / cal_s  # offered load relative to plain capacity
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
-    arrivals -= arrivals[0]
-    jp = jobs()
-    plain_s, plain_lat = serve.run_stream(plain, jp, list(arrivals))
-    plain.close()
-    served = serve.GpuServer(tables, hbm_budget=200e9, workers=args.serve_workers, batching=True)
+    batch_s = args.serve_batch_ms * 1e-3 if args.serve_batch_ms > 0 else float("inf")
+    served = serve.GpuServer(tables, hbm_budget=200e9, workers=args.serve_workers, batching=True,
+                             max_batch_s=batch_s)
     serve.run_stream(served, jobs()[:8], [0.0] * 8)  # warm-up
-    js = jobs()
-    served_s, served_lat = serve.run_stream(served, js, list(arrivals))
-    batches = served.batches
+
+    def at_load(load):
+        rate = load * njobs / cal_s  # offered load relative to plain capacity
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+        arrivals -= arrivals[0]
+        jp = jobs()
+        plain_s, plain_lat = serve.run_stream(plain, jp, list(arrivals))
+        b0 = served.batches
+        js = jobs()
+        served_s, served_lat = serve.run_stream(served, js, list(arrivals))
+        same = all((a.result.tobytes() == b.result.tobytes()) for a, b in zip(jp, js))
+        return rate, plain_s, plain_lat, served_s, served_lat, served.batches - b0, same
+
+    rate, plain_s, plain_lat, served_s, served_lat, batches, same = at_load(args.serve_load)
+    low = None
+    if args.serve_load2 > 0:
+        r2, p2, pl2, s2, sl2, nb2, same2 = at_load(args.serve_load2)
+        same = same and same2
+        low = {"offered_load_x_plain_capacity": args.serve_load2, "offered_rate_jobs_per_s": r2,
+               "plain_jobs_per_s": njobs / p2, "served_jobs_per_s": njobs / s2,
+               "plain_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(pl2).items()},
+               "served_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(sl2).items()},
+               "served_batches": nb2}
+    plain.close()
     served.close()
-    same = all((a.result.tobytes() == b.result.tobytes()) for a, b in zip(jp, js))
     for t in tables:
         t.close()
     return {"jobs": njobs, "offered_rate_jobs_per_s": rate, "plain_capacity_jobs_per_s": njobs / cal_s,
@@ -405,6 +425,8 @@ def bench_served(args):
             "plain_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(plain_lat).items()},
             "served_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(served_lat).items()},
             "served_batches": batches, "workers": args.serve_workers, "hbm_budget_gb": 200,
+            "offered_load_x_plain_capacity": args.serve_load, "max_batch_ms": args.serve_batch_ms,
+            "below_saturation": low,
             "results_equal_plain": bool(same),
             "note": "host inputs (H2D/D2H inside each job); MSM scalars are slices of one seeded pool"}
 

@@ -3,10 +3,12 @@
 `GpuServer` is a continuous-batching request loop:
   - the MSM base-point tables stay resident in HBM (MsmContext objects);
   - every worker thread owns a HIP stream and an MSM workspace, and
-    repeatedly takes all waiting jobs that fit the free HBM budget and runs
-    them as ONE batch (grouped NTT launches + one MSM launch sequence), so
-    requests that arrive while other batches are on the GPU are admitted as
-    soon as a worker and memory free up, and their work overlaps on the device;
+    repeatedly takes the waiting jobs, in arrival order, that fit the free HBM
+    budget and an estimated amount of work per batch (`max_batch_s`, so that
+    results stream out instead of finishing together), and runs them as ONE
+    batch (grouped NTT launches + one MSM launch sequence); requests that
+    arrive while other batches are on the GPU are admitted as soon as a worker
+    and memory free up, and their work overlaps on the device;
   - the plain form is the same loop with one worker and one job per batch.
 
 No reference counterpart: the reference serialises the device and overlaps
@@ -60,6 +62,17 @@ class Job:
         w = -(-253 // c)
         return 32 * n + 16 * w * n + 4 * 192 * w * (1 << (c - 1)) + (1 << 26)
 
+    def est_s(self):
+        """Rough time of this job inside a batch: host<->device copies of its
+        inputs/outputs (pageable, ~25 GB/s) plus device work (MSM ~0.17 ns per
+        bucket addition, N per window; NTT ~15 ps per element)."""
+        n = 1 << self.log_n
+        if self.kind == NTT:
+            return 16 * n / 25e9 + 15e-12 * n
+        c = min(20, max(8, self.log_n - 6))
+        w = -(-254 // c)
+        return 32 * n / 25e9 + 0.17e-9 * n * w + 2e-4
+
     def wait(self, timeout=None):
         self._event.wait(timeout)
         return self
@@ -89,10 +102,32 @@ class _DevicePool:
         self.free.clear()
 
 
+def admit(queue, in_use, budget, max_jobs, max_batch_s):
+    """Pop the next batch from the head of `queue` (a deque of Jobs, arrival
+    order): jobs are taken while the batch's HBM need fits `budget - in_use`,
+    its estimated work (Job.est_s) stays within `max_batch_s` and it holds at
+    most `max_jobs` jobs; the first job is taken whenever it fits the budget,
+    whatever its estimated work.  Returns (batch, bytes needed); an empty batch
+    when the head job does not fit the free budget."""
+    batch, need, work = [], 0.0, 0.0
+    while queue and len(batch) < max_jobs:
+        b = queue[0].hbm_bytes()
+        t = queue[0].est_s()
+        if in_use + need + b > budget:
+            break
+        if batch and work + t > max_batch_s:
+            break
+        batch.append(queue.popleft())
+        need += b
+        work += t
+    return batch, need
+
+
 class GpuServer:
-    def __init__(self, tables, hbm_budget=200e9, workers=2, max_batch_jobs=256, batching=True):
+    def __init__(self, tables, hbm_budget=200e9, workers=2, max_batch_jobs=256, batching=True, max_batch_s=0.1):
         self.tables = tables
         self.budget = float(hbm_budget)
+        self.max_batch_s = float(max_batch_s)
         self.in_use = 0.0
         self.batching = batching
         self.max_batch = max_batch_jobs if batching else 1
@@ -123,25 +158,18 @@ class GpuServer:
 
     # ------------------------------------------------------------- worker
     def _take(self):
-        """Admit waiting jobs in arrival order while they fit the free budget."""
+        """Admit waiting jobs (see `admit`); waits while nothing fits."""
         with self.cv:
-            while not self.queue and not self.stop:
-                self.cv.wait()
-            if not self.queue:
-                return None
-            batch, need = [], 0.0
-            while self.queue and len(batch) < self.max_batch:
-                b = self.queue[0].hbm_bytes()
-                if batch and self.in_use + need + b > self.budget:
-                    break
-                if not batch and self.in_use + b > self.budget:
-                    # wait for memory to free up
-                    self.cv.wait(0.01)
-                    continue
-                batch.append(self.queue.popleft())
-                need += b
-            self.in_use += need
-            return batch, need
+            while True:
+                while not self.queue and not self.stop:
+                    self.cv.wait()
+                if not self.queue:
+                    return None
+                batch, need = admit(self.queue, self.in_use, self.budget, self.max_batch, self.max_batch_s)
+                if batch:
+                    self.in_use += need
+                    return batch, need
+                self.cv.wait(0.01)  # the head job does not fit the free budget yet
 
     def _release(self, need):
         with self.cv:
@@ -249,4 +277,4 @@ def percentiles(values, ps=(50, 99)):
     return {f"p{p}": float(np.percentile(v, p)) for p in ps}
 
 
-__all__ = ["GpuServer", "Job", "NTT", "MSM", "run_stream", "percentiles", "G1_PROJ_DTYPE"]
+__all__ = ["GpuServer", "Job", "NTT", "MSM", "admit", "run_stream", "percentiles", "G1_PROJ_DTYPE"]
