@@ -49,7 +49,7 @@ def test_random_vs_oracle(log_n):
 
 @pytest.mark.parametrize("log_n", [21, 22, 23, 24])
 def test_large_vs_oracle(log_n):
-    # 2^24 as a 4096 x 4096 4-step, seed 1, bit-exact vs the CPU reference
+    # the 2^24 workload: a 4096 x 4096 4-step, seed 1, bit-exact vs the CPU reference
     _say(f"ntt large log_n={log_n}")
     a = _rand(1 << log_n, 1 if log_n == 24 else log_n)
     with zntt.NttGpuDriver(log_n) as d:

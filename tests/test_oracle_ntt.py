@@ -96,7 +96,7 @@ def test_brute_force(ntt_golden):
 
 
 def test_four_step_equals_direct_64x64():
-    # n = 2^12 as a 64 x 64 4-step on the software model
+    # the 2^12 workload: n = 4096 as a 64 x 64 4-step on the software model
     rng = np.random.default_rng(0)
     a = rng.integers(0, P, size=4096, dtype=np.uint64)
     assert np.array_equal(oracle.ntt_four_step(a, 6), oracle.ntt(a))
