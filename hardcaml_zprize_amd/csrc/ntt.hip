@@ -1,8 +1,8 @@
 // Goldilocks NTT on gfx950: plans, kernels and the NTT half of the C ABI.
 //
 // A transform of n = 2^k points:
-//   k <= 12: one workgroup holds the whole vector in LDS (a single pass);
-//   k  > 12: the 4-step of libs/hardcaml_ntt/src/reference_model.ml:92-105 with
+//   k <= 9: one workgroup holds the whole vector in LDS (a single pass);
+//   k  > 9: the 4-step of libs/hardcaml_ntt/src/reference_model.ml:92-105 with
 //            n = n1 * n2 (n1 = 2^ceil(k/2)), as the FPGA's two phases
 //            (zprize/ntt/host/driver.cpp:201-231):
 //     pass 1  column DFTs of length n1 over the n1 x n2 row-major matrix,
@@ -106,18 +106,26 @@ struct NttShape {
   unsigned log_n, log_n1, log_n2;
 };
 
+// Up to 2^9 points one workgroup does the whole transform in LDS; above, the
+// two passes spread the work over many workgroups.
 static NttShape shape_of(unsigned log_n) {
   NttShape s;
   s.log_n = log_n;
-  if (log_n <= 12) {
+  if (log_n <= 9) {
     s.log_n1 = log_n;
     s.log_n2 = 0;
+  } else if (log_n >= 21) {
+    s.log_n1 = 12;  // 4096-point columns in registers (pass 1)
+    s.log_n2 = log_n - 12;
   } else {
     s.log_n1 = (log_n + 1) / 2;
     s.log_n2 = log_n - s.log_n1;
   }
   return s;
 }
+
+// Two-pass transforms keep the 4-step intermediate in an n-element scratch vector.
+static bool needs_scratch(unsigned log_n) { return shape_of(log_n).log_n2 > 0; }
 
 // Tile widths (sequences per workgroup) of the 4096-point kernels, per pass:
 // ZP_NTT_TILE_COLS / ZP_NTT_TILE_ROWS = 4 or 8 (ZP_NTT_TILE sets both).
@@ -137,16 +145,18 @@ struct JobTabs {
   unsigned jobs = 1;
 };
 
+// Pass 1 with 4096-point columns: n2 columns (n2 >= T, a power of two), so
+// 2^24 (n2 = 4096) and 2^23 (n2 = 2048) share the register-resident kernel.
 template <int T>
 static int launch_4096_cols(const NttTables *t, uint64_t *d_in, uint64_t *d_out, int dir, hipStream_t stream,
-                            const JobTabs &jt) {
+                            const JobTabs &jt, uint64_t n2 = 4096) {
   Ntt4096Args a{};
   a.n1 = 4096;
-  a.n2 = 4096;
+  a.n2 = n2;
   a.tw4096 = t->d_4096[dir];
   a.tw_lo = t->d_lo[dir];
   a.tw_hi = t->d_hi[dir];
-  a.tiles = 4096 / T;
+  a.tiles = (uint32_t)(n2 / T);
   a.in = d_in;
   a.out = d_out;
   a.in_tab = jt.data;
@@ -247,8 +257,8 @@ static void tile_widths() {
 
 // One pass of an in-place transform (pass 1 = column DFTs + 4-step twiddle,
 // data -> scratch; pass 2 = row DFTs, scratch -> data transposed).  A length
-// that is a single DFT (log_n <= 12) has only pass 1, which works in place.
-// scratch must hold n elements when log_n > 12.
+// that is a single DFT (log_n <= 9) has only pass 1, which works in place.
+// scratch must hold n elements otherwise (needs_scratch).
 static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, int pass,
                        hipStream_t stream, const JobTabs &jt = JobTabs()) {
   const NttShape sh = shape_of(t->log_n);
@@ -259,6 +269,15 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
                               : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt);
     return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
                             : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
+  }
+  if (pass == 1 && sh.log_n1 == 12 && sh.log_n2 >= 3) {
+    // 4096-point columns in registers; pass 2 (rows shorter than 4096) stays
+    // on the LDS kernel, which takes the lazy scratch values (canonicalized
+    // on load).
+    tile_widths();
+    const uint64_t n2 = 1ull << sh.log_n2;
+    return g_tile_cols == 4 ? launch_4096_cols<4>(t, d_data, d_scratch, dir, stream, jt, n2)
+                            : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt, n2);
   }
   const uint64_t n = 1ull << sh.log_n;
   const uint64_t scale = dir ? t->n_inv : 1;
@@ -392,7 +411,7 @@ static int create_plan(unsigned log_n, unsigned num_buffers, bool reverse, zp_nt
   p->n = 1ull << log_n;
   p->reverse = reverse;
   int st = make_tables(log_n, &p->tables);
-  if (st == ZP_OK && (log_n > 12 || reverse) && hipMalloc(&p->d_scratch, p->n * 8) != hipSuccess)
+  if (st == ZP_OK && (needs_scratch(log_n) || reverse) && hipMalloc(&p->d_scratch, p->n * 8) != hipSuccess)
     st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipMalloc(&p->d_work, p->n * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
   if (st == ZP_OK && hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) st = ZP_ERR_HIP;
@@ -644,7 +663,7 @@ extern "C" int zp_ntt_batch_create(uint64_t *const *d_ptrs, const unsigned *log_
     if (b->groups.empty() || b->groups.back().log_n != log_ns[j] || b->groups.back().dir != dirs[j])
       b->groups.push_back(NttGroup{log_ns[j], dirs[j], 0, i});
     b->groups.back().jobs++;
-    if (log_ns[j] > 12) scratch_need += 1ull << log_ns[j];
+    if (needs_scratch(log_ns[j])) scratch_need += 1ull << log_ns[j];
   }
   int st = ZP_OK;
   if (scratch_need && hipMalloc(&b->d_scratch, scratch_need * 8) != hipSuccess) st = ZP_ERR_OUT_OF_MEMORY;
@@ -652,7 +671,7 @@ extern "C" int zp_ntt_batch_create(uint64_t *const *d_ptrs, const unsigned *log_
   for (size_t i = 0; i < count; i++) {
     const size_t j = order[i];
     tabs[i] = d_ptrs[j];
-    if (log_ns[j] > 12 && b->d_scratch) {
+    if (needs_scratch(log_ns[j]) && b->d_scratch) {
       tabs[count + i] = b->d_scratch + off;
       off += 1ull << log_ns[j];
     }

@@ -1,8 +1,8 @@
 // Sub-DFT kernels shared by the single-pass and the 4-step transforms.
 //
 // k_dft_lds: each workgroup loads 2^log_T sequences of length L = 2^log_L
-// (element i of sequence s at in[s * in_seq_stride + i * in_elem_stride]) into
-// LDS in bit-reversed order, runs the radix-2 DIT stages of
+// (element i of sequence s at in[s * in_seq_stride + i * in_elem_stride],
+// reduced mod p) into LDS in bit-reversed order, runs the radix-2 DIT stages of
 // zprize/ntt/host/reference.cpp:109-136, optionally multiplies output k of
 // sequence s by w_n^(s * k) (the 4-step twiddle, reference_model.ml:75-90) and
 // by a scale, and stores element k at out[s * out_seq_stride + k * out_elem_stride].
@@ -33,9 +33,14 @@ struct DftArgs {
 static constexpr int DFT_THREADS = 256;
 static constexpr unsigned DFT_LDS_LOG_ELEMS = 13;  // 64 KiB of u64
 
-// Sequences per workgroup: fill 2^13 elements of LDS, at most the number of sequences.
+// Sequences per workgroup: at most 2^13 elements of LDS, and few enough that
+// a transform still spreads over >= 2^8 workgroups (one per CU) when it has
+// that many sequences; small transforms are latency-bound, and the column
+// reads that a narrow tile makes uncoalesced stay in L2 at these sizes.
 static inline unsigned dft_log_T(unsigned log_L, unsigned log_num_seq) {
   unsigned lt = log_L >= DFT_LDS_LOG_ELEMS ? 0 : DFT_LDS_LOG_ELEMS - log_L;
+  const unsigned spread = log_num_seq > 8 ? log_num_seq - 8 : 0;
+  if (lt > spread) lt = spread;
   return lt < log_num_seq ? lt : log_num_seq;
 }
 
@@ -63,7 +68,10 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft_lds(DftArgs a) {
       s = idx & (T - 1);
       i = idx >> a.log_T;
     }
-    uint64_t v = a.in[(seq0 + s) * a.in_seq_stride + (uint64_t)i * a.in_elem_stride];
+    // Inputs >= p count as their residue (user vectors, and the lazy scratch
+    // values the register-resident column kernel leaves for pass 2); the
+    // stages below need canonical operands.
+    const uint64_t v = gl_canon(a.in[(seq0 + s) * a.in_seq_stride + (uint64_t)i * a.in_elem_stride]);
     tile[(s << a.log_L) + bitrev_bits(i, a.log_L)] = v;
   }
   __syncthreads();

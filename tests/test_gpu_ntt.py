@@ -59,6 +59,24 @@ def test_large_vs_oracle(log_n):
         assert np.array_equal(back, a)
 
 
+@pytest.mark.parametrize("log_n", [3, 10, 13, 17, 23, 24])
+def test_noncanonical_inputs_count_as_residues(log_n):
+    """Inputs in [p, 2^64) transform as their residue mod p at every size
+    (the single-pass, LDS and register-resident paths alike)."""
+    _say(f"ntt noncanonical log_n={log_n}")
+    n = 1 << log_n
+    rng = np.random.default_rng(500 + log_n)
+    a = rng.integers(0, P, size=n, dtype=np.uint64)
+    big = a.copy()
+    sel = rng.random(n) < 0.5
+    big[sel] = (a[sel] % np.uint64(0xFFFFFFFF)) + np.uint64(P)  # p + r, r < 2^32 - 1: still a u64, = r mod p
+    red = big.copy()
+    red[sel] = big[sel] - np.uint64(P)
+    with zntt.NttGpuDriver(log_n) as d:
+        assert np.array_equal(d.simple_evaluate(big, zntt.FORWARD), oracle.ntt(red))
+        assert np.array_equal(d.simple_evaluate(big, zntt.INVERSE), oracle.ntt(red, inverse=True))
+
+
 def test_edge_values():
     _say("ntt edge values")
     with zntt.NttGpuDriver(14) as d:
