@@ -134,6 +134,7 @@ class GpuServer:
         self.queue = deque()
         self.cv = threading.Condition()
         self.stop = False
+        self.failed = None  # set when a worker could not start; the server then refuses work
         self.batches = 0
         self.windows = autotune.load()  # size class -> tuned MSM window (empty = library default)
         self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
@@ -145,6 +146,8 @@ class GpuServer:
         if job.hbm_bytes() > self.budget:
             raise ValueError(f"job {job.id} needs {job.hbm_bytes()} bytes > HBM budget {self.budget}")
         with self.cv:
+            if self.failed is not None:
+                raise RuntimeError(f"server unavailable: a worker failed to start ({self.failed})")
             self.queue.append(job)
             self.cv.notify()
         return job
@@ -176,11 +179,32 @@ class GpuServer:
             self.in_use -= need
             self.cv.notify_all()
 
+    def _fail(self, err):
+        """A worker could not start: fail every waiting job (so no caller
+        waits forever) and refuse new ones."""
+        with self.cv:
+            if self.failed is None:
+                self.failed = err
+            pending = list(self.queue)
+            self.queue.clear()
+            self.cv.notify_all()
+        for j in pending:
+            j.error = err
+            j.t_done = time.perf_counter()
+            j._event.set()
+
     def _worker(self):
-        s = ctypes.c_void_p()
-        call("zp_stream_create", ctypes.byref(s))
-        stream = s.value
-        ws = MsmWorkspace()
+        stream = ws = None
+        try:
+            s = ctypes.c_void_p()
+            call("zp_stream_create", ctypes.byref(s))
+            stream = s.value
+            ws = MsmWorkspace()
+        except Exception as e:
+            if stream is not None:
+                lib().zp_stream_destroy(ctypes.c_void_p(stream))
+            self._fail(e)
+            return
         pool = _DevicePool()
         try:
             while True:

@@ -69,3 +69,27 @@ def test_job_count_cap(cap):
     q = deque(ntt(i, 10) for i in range(5))
     batch, _ = serve.admit(q, 0.0, 200e9, cap, float("inf"))
     assert ids(batch) == list(range(cap))
+
+
+def test_worker_start_failure_fails_waiting_jobs(monkeypatch):
+    """A worker that cannot create its stream fails the jobs already queued
+    (their waiters return with the error instead of blocking forever) and the
+    server refuses later submissions.  The failure is forced on the host."""
+    import threading
+
+    release = threading.Event()
+
+    def failing_call(name, *args):
+        release.wait(10)
+        raise RuntimeError(f"{name}: no device")
+
+    monkeypatch.setattr(serve, "call", failing_call)
+    srv = serve.GpuServer({}, workers=1)
+    queued = srv.submit(ntt(0, 10))  # the worker is still starting
+    release.set()
+    assert queued._event.wait(10), "queued job was never completed"
+    assert isinstance(queued.error, RuntimeError)
+    srv.threads[0].join(10)
+    with pytest.raises(RuntimeError, match="worker failed to start"):
+        srv.submit(ntt(1, 10))
+    srv.close()
