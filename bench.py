@@ -32,6 +32,9 @@ def parse():
     ap.add_argument("--msm-log-n", type=int, default=26)
     ap.add_argument("--msm-steps", type=int, default=3)
     ap.add_argument("--msm-warmup", type=int, default=1)
+    ap.add_argument("--msm-options", default="precompute=auto",
+                    help="msm_init option string of the 2^N table ('' = default options; otherwise the "
+                         "default-option table is timed too, as plain_table)")
     ap.add_argument("--serve-jobs", type=int, default=120)
     ap.add_argument("--serve-load", type=float, default=6.0)
     ap.add_argument("--serve-workers", type=int, default=2)
@@ -181,7 +184,7 @@ def rand_scalars(rng, n, nonzero=False):
     return out
 
 
-def make_table(zmsm, rng, log_n):
+def make_table(zmsm, rng, log_n, options=""):
     """P_i = k_i G generated on the GPU, loaded as a resident MSM table."""
     import numpy as np
 
@@ -194,7 +197,7 @@ def make_table(zmsm, rng, log_n):
     from hardcaml_zprize_amd._lib import call
     call("zp_device_synchronize")
     t1 = time.perf_counter()
-    ctx = zmsm.MsmContext((dp.ptr, n))
+    ctx = zmsm.MsmContext((dp.ptr, n), options)
     t2 = time.perf_counter()
     dk.free()
     dp.free()
@@ -203,19 +206,33 @@ def make_table(zmsm, rng, log_n):
 
 
 def bench_msm(args, probe):
+    """The 2^N MSM on a table loaded with --msm-options (default: with its
+    window-shifted copies, "precompute=auto"), then the same table loaded with
+    the default options for comparison."""
+    res = bench_msm_table(args, probe, args.msm_options)
+    res["options"] = args.msm_options
+    if args.msm_options:
+        plain = bench_msm_table(args, probe, "", steps=max(1, args.msm_steps - 1), multi=False)
+        res["plain_table"] = {"options": "", "ms": plain["ms"], "ms_msm_mult_host_scalars":
+                              plain["ms_msm_mult_host_scalars"], "init_s": plain["init_s"]}
+    return res
+
+
+def bench_msm_table(args, probe, options, steps=None, multi=True):
     import numpy as np
 
     from hardcaml_zprize_amd import msm as zmsm
 
     log_n = args.msm_log_n
     n = 1 << log_n
+    steps = args.msm_steps if steps is None else steps
     rng = np.random.default_rng(args.seed + 1000)
     log(f"MSM 2^{log_n}: generating base points on the GPU")
-    ctx, gen_s, init_s = make_table(zmsm, rng, log_n)
-    log(f"MSM points generated in {gen_s:.2f} s, msm_init (resident precompute) {init_s:.2f} s")
+    ctx, gen_s, init_s = make_table(zmsm, rng, log_n, options)
+    log(f"MSM points generated in {gen_s:.2f} s, msm_init ({options or 'default options'}) {init_s:.2f} s")
     dsc = zmsm.DeviceBuffer(32 * n)
     dev_ms, host_ms = [], []
-    for call_i in range(args.msm_warmup + args.msm_steps):
+    for call_i in range(args.msm_warmup + steps):
         sc = rand_scalars(np.random.default_rng([args.seed, call_i]), n)  # fresh per call
         t0 = time.perf_counter()
         ctx.mult(sc, 1)  # msm_mult with host scalars (H2D included)
@@ -232,12 +249,14 @@ def bench_msm(args, probe):
     # msm_mult over several batches (the harness's num_batches): the upload of
     # batch b + 1 overlaps the computation of batch b.
     batches = 2
-    sc = np.concatenate([rand_scalars(np.random.default_rng([args.seed, 99, b]), n) for b in range(batches)])
-    t0 = time.perf_counter()
-    ctx.mult(sc, batches)
-    multi_ms = (time.perf_counter() - t0) * 1e3 / batches
-    del sc
-    log(f"MSM msm_mult x{batches} batches: {multi_ms:.1f} ms per batch")
+    multi_ms = None
+    if multi:
+        sc = np.concatenate([rand_scalars(np.random.default_rng([args.seed, 99, b]), n) for b in range(batches)])
+        t0 = time.perf_counter()
+        ctx.mult(sc, batches)
+        multi_ms = (time.perf_counter() - t0) * 1e3 / batches
+        del sc
+        log(f"MSM msm_mult x{batches} batches: {multi_ms:.1f} ms per batch")
     ctx.close()
     ms = float(np.median(dev_ms))
     work = 140 * n * 288  # reference algorithm's v_mad_u64_u32 lane-ops (20 windows x 7 Fq muls x 288 MACs)

@@ -99,6 +99,10 @@ struct MsmOptions {
   int sort = 0;
   bool bad_sort = false;
   int sort_lbits = -1;  // two-level: low bucket bits sorted in pass B, -1 = auto
+  // msm_init only: also keep the window-shifted table 2^(c w) P_i for the
+  // uniform layout of c bits ("precompute=<c>" or "precompute=auto"); 0 = off.
+  int precompute = 0;
+  bool bad_precompute = false;
 };
 
 static MsmOptions parse_options(const char *s, size_t len) {
@@ -129,6 +133,9 @@ static MsmOptions parse_options(const char *s, size_t len) {
         o.bad_sort = o.sort < 0;
       } else if (k == "sort_lbits") {
         o.sort_lbits = atoi(v.c_str());
+      } else if (k == "precompute") {
+        o.precompute = v == "auto" ? -1 : v == "off" ? 0 : atoi(v.c_str());
+        o.bad_precompute = v != "auto" && v != "off" && (o.precompute < 8 || o.precompute > 24);
       }
     }
     pos = end + 1;
@@ -143,6 +150,7 @@ static int validate_options(const MsmOptions &o, const char *who) {
   ZP_CHECK(!o.bad_coords, ZP_ERR_INVALID_ARG, "%s: coords= must be canonical or montgomery", who);
   ZP_CHECK(!o.bad_sort, ZP_ERR_INVALID_ARG, "%s: sort= must be auto, lds, msd or atomic", who);
   ZP_CHECK(o.sort_lbits <= 12, ZP_ERR_INVALID_ARG, "%s: sort_lbits=%d > 12", who, o.sort_lbits);
+  ZP_CHECK(!o.bad_precompute, ZP_ERR_INVALID_ARG, "%s: precompute= must be auto, off or a width in [8, 24]", who);
   return ZP_OK;
 }
 
@@ -236,6 +244,50 @@ __global__ void k_pre_c(const F *u_in, const F *v_in, const F *inv_den, const ui
     out[i] = pre_pack(pre);
     conv[i] = f != 0;
   }
+}
+
+// Window-shifted table (init option "precompute"): entry i * W + w holds
+// 2^(c w) P_i in the packed precomputed form, so that every window of a
+// c-bit layout adds into one shared bucket set.  Entry w is made from entry
+// w - 1 by c doublings and one affine normalisation (batch inversion of Z).
+// k_shift_a: entry w - 1 of points [i0, i0 + m) -> X, Y, Z after c doublings.
+// A Z of zero (a point of small order whose multiple has no affine
+// twisted-Edwards image) sets *bad, and the caller drops the table.
+__global__ void k_shift_a(const PackedPre *tab, uint64_t i0, uint64_t m, uint32_t W, uint32_t w, uint32_t c,
+                          const uint8_t *conv, FConsts K, F *X, F *Y, F *Z, int *bad) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = i0 + k;
+    if (!conv[i]) {  // never referenced: its digits are zero
+      X[k] = f_zero();
+      Y[k] = K.one;
+      Z[k] = K.one;
+      continue;
+    }
+    Xyzt e = ext_from_pre(pre_unpack(tab[i * W + (w - 1)]), false, K);
+    for (uint32_t d = 0; d < c; d++) ec_dbl(e, K);
+    if (f_is_zero_canon(e.z)) {
+      atomicOr(bad, 1);
+      e.z = K.one;
+    }
+    X[k] = e.x;
+    Y[k] = e.y;
+    Z[k] = e.z;
+  }
+}
+
+// k_shift_c: X, Y and 1/Z -> entry w of points [i0, i0 + m).
+__global__ void k_shift_c(const F *X, const F *Y, const F *Zinv, uint64_t i0, uint64_t m, uint32_t W, uint32_t w,
+                          FConsts K, PackedPre *tab) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+    const F x = f_mul(X[k], Zinv[k]), y = f_mul(Y[k], Zinv[k]);
+    tab[(i0 + k) * W + w] = pre_pack(pre_from_affine(x, y, K));
+  }
+}
+
+// Entry 0 of every point: the table itself.
+__global__ void k_shift_copy0(const PackedPre *pts, uint64_t n, uint32_t W, PackedPre *tab) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    tab[i * W] = pts[i];
 }
 
 // Fixed-base comb: acc = sum_j T[j][byte_j(k)] with T[j][m] = m 2^(8j) G.
@@ -367,6 +419,8 @@ struct zp_msm_workspace {
   uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
   uint32_t *ubucket = nullptr, *vals = nullptr, *gstart = nullptr, *pt0 = nullptr, *b0 = nullptr;
   Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
+  Xyzt *winpart = nullptr;  // slice sums of windows with many groups
+  size_t cap_winpart = 0;
   MsmJobDev *jobs = nullptr;
   WinDev *wins = nullptr;
   uint8_t *scan_tmp = nullptr;
@@ -377,7 +431,8 @@ struct zp_msm_workspace {
 
   void release() {
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
-                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0};
+                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
+                  winpart};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
@@ -488,6 +543,10 @@ struct zp_msm_context {
   size_t num_nonconv = 0;
   uint32_t *d_nc_idx = nullptr;  // every group's indices, group after group
   uint32_t *d_nc_off = nullptr;  // nc_groups.size() + 1 offsets into d_nc_idx
+  // Window-shifted table (init option "precompute"): d_shift[i * shift_W + w]
+  // = 2^(shift_c w) P_i; null when not requested.
+  PackedPre *d_shift = nullptr;
+  int shift_c = 0, shift_W = 0;
   zp_msm_workspace ws;
   uint64_t *d_scalars = nullptr;  // staging for host scalars
   size_t cap_scalars = 0;
@@ -612,6 +671,19 @@ static void finalize(const HostLayout &L, const Xyzt *win, const std::vector<NcG
   memcpy(out->z.data, wz.w, 48);
 }
 
+// Bucket additions of an n-point job: n per window, plus about 2.6 (full
+// additions count ~1.3 mixed ones) per bucket of the running-sum reduction;
+// with shared buckets every window adds into the largest window's buckets.
+static double msm_cost(uint64_t n, const HostLayout &L, bool shared) {
+  double buckets = 0;
+  uint32_t maxnb = 0;
+  for (int w = 0; w < L.W; w++) {
+    buckets += L.nb[w];
+    maxnb = std::max(maxnb, L.nb[w]);
+  }
+  return (double)n * L.W + 2.6 * (shared ? (double)maxnb : buckets);
+}
+
 // Host description of one job of a batch.
 struct HostJob {
   zp_msm_context *ctx;
@@ -632,6 +704,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   std::vector<SegDev> segs;
   uint64_t K = 0, entries = 0, points = 0, counters = 0, n_max = 0;
   uint32_t max_nb = 0;
+  bool any_pre = false;
   for (size_t j = 0; j < count; j++) {
     const HostJob &h = jobs[j];
     MsmOptions o = h.ctx->opts;
@@ -644,42 +717,98 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       o.reference = false;
     }
     layouts[j] = choose_layout(o, h.n);
-    const HostLayout &L = layouts[j];
+    const zp_msm_context *cx = h.ctx;
+    // A table with window-shifted copies serves the job as ONE window of
+    // n * W virtual points (digit w of point i = virtual point i * W + w, the
+    // table entry 2^(c w) P_i) when that costs fewer bucket additions than the
+    // per-window layout (small prefixes of a big table do not: one shared
+    // bucket set of 2^(c-1) buckets outweighs their points).
+    bool pre = false;
+    HostLayout LS;
+    if (cx->d_shift && !h.window && !bopts.window && !bopts.reference && h.n) {
+      LS = layout_uniform(cx->shift_c);
+      layout_finish(LS);
+      pre = msm_cost(h.n, LS, true) < msm_cost(h.n, layouts[j], false);
+    }
     MsmJobDev &d = jd[j];
-    d.pts = h.ctx->d_pts;
-    d.conv = h.ctx->d_conv;
+    d.conv = cx->d_conv;
     d.scalars = h.d_scalars;
-    d.n = (uint32_t)h.n;
     d.pt0 = (uint32_t)points;
     d.win0 = (uint32_t)wins.size();
-    d.W = (uint32_t)L.W;
     d.bucket0 = (uint32_t)K;
+    d.dn = (uint32_t)h.n;
+    d.pre = pre;
     pt0[j] = d.pt0;
     b0[j] = d.bucket0;
-    for (int w = 0; w < L.W; w++) {
-      WinDev wd;
-      wd.job = (uint32_t)j;
-      wd.off = (uint32_t)L.off[w];
-      wd.width = (uint32_t)L.width[w];
-      wd.nb = L.nb[w];
-      wd.base = (uint32_t)K;
-      wd.top = w == L.W - 1;
-      K += L.nb[w];
-      max_nb = std::max(max_nb, L.nb[w]);
-      wins.push_back(wd);
+    if (pre) {
+      any_pre = true;
+      uint32_t nbv = 0;
+      for (int w = 0; w < LS.W; w++) nbv = std::max(nbv, LS.nb[w]);
+      WinDev v{};
+      v.job = (uint32_t)j;
+      v.width = (uint32_t)cx->shift_c;
+      v.nb = nbv;
+      v.base = (uint32_t)K;
+      wins.push_back(v);
+      K += nbv;
+      max_nb = std::max(max_nb, nbv);
+      d.W = 1;
+      // the layout's windows, for the digits only (no buckets)
+      d.dwin0 = (uint32_t)wins.size();
+      d.dW = (uint32_t)LS.W;
+      for (int w = 0; w < LS.W; w++) {
+        WinDev wd{};
+        wd.job = (uint32_t)j;
+        wd.off = (uint32_t)LS.off[w];
+        wd.width = (uint32_t)LS.width[w];
+        wd.base = (uint32_t)K;
+        wd.top = w == LS.W - 1;
+        wins.push_back(wd);
+      }
+      d.pts = cx->d_shift;
+      d.n = (uint32_t)(h.n * LS.W);
+      HostLayout L1;  // the result is the one window's sum
+      L1.W = 1;
+      L1.width[0] = cx->shift_c;
+      L1.nb[0] = nbv;
+      layouts[j] = L1;
+      // One window of up to 2^23 buckets: the two-level sort splits it into
+      // up to 2^14 groups (see below) and takes 2^18-entry chunks, so that
+      // pass A still writes runs of ~16 entries per group and chunk, and
+      // pass B's groups stay small enough for L2 to merge its scattered writes.
+      d.chunks = (uint32_t)std::min<uint64_t>(16384, std::max<uint64_t>(1, (uint64_t)d.n >> 18));
+    } else {
+      const HostLayout &L = layouts[j];
+      d.pts = cx->d_pts;
+      d.n = (uint32_t)h.n;
+      d.W = (uint32_t)L.W;
+      d.dwin0 = d.win0;
+      d.dW = d.W;
+      for (int w = 0; w < L.W; w++) {
+        WinDev wd{};
+        wd.job = (uint32_t)j;
+        wd.off = (uint32_t)L.off[w];
+        wd.width = (uint32_t)L.width[w];
+        wd.nb = L.nb[w];
+        wd.base = (uint32_t)K;
+        wd.top = w == L.W - 1;
+        K += L.nb[w];
+        max_nb = std::max(max_nb, L.nb[w]);
+        wins.push_back(wd);
+      }
+      // sorted path: C chunks of <= chunk_len points per window.  Chunks of
+      // 2^16 points (up to 256 of them) keep each scatter block's output small
+      // enough for L2 to merge its partial-line writes (measured at 2^26:
+      // 64 chunks 179 ms, 128 174, 256 170-175, 512 174).
+      d.chunks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, h.n >> 16));
     }
-    // sorted path: C chunks of <= chunk_len points per window.  Chunks of
-    // 2^16 points (up to 256 of them) keep each scatter block's output small
-    // enough for L2 to merge its partial-line writes (measured at 2^26:
-    // 64 chunks 179 ms, 128 174, 256 170-175, 512 174).
-    d.chunks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, h.n >> 16));
-    d.chunk_len = (uint32_t)std::max<uint64_t>(1, (h.n + d.chunks - 1) / d.chunks);
+    d.chunk_len = (uint32_t)std::max<uint64_t>(1, ((uint64_t)d.n + d.chunks - 1) / d.chunks);
     d.dig0 = entries;
-    for (int w = 0; w < L.W; w++)
-      for (uint32_t c = 0; c < d.chunks; c++) segs.push_back(SegDev{(uint32_t)j, d.win0 + (uint32_t)w, c});
+    for (uint32_t w = 0; w < d.W; w++)
+      for (uint32_t c = 0; c < d.chunks; c++) segs.push_back(SegDev{(uint32_t)j, d.win0 + w, c});
     points += h.n;
-    entries += (uint64_t)h.n * L.W;
-    n_max = std::max<uint64_t>(n_max, h.n);
+    entries += (uint64_t)d.n * d.W;
+    n_max = std::max<uint64_t>(n_max, d.n);
   }
   // Sort: the single-level LDS histogram holds one window's buckets (<= 2^15
   // u32 = 128 KiB); the two-level sort takes any width (pass A: <= 2^12
@@ -689,6 +818,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   if (force_atomic && atoi(force_atomic)) sort = 3;
   if (sort == 0) sort = max_nb <= 32768 ? 1 : 2;
   if (sort == 1 && max_nb > 32768) sort = 2;
+  if (any_pre) sort = 2;  // a one-window job over shifted tables needs the two-level sort
   const bool sorted_path = sort == 1, msd_path = sort == 2;
   // Two-level split per window: group = bucket >> lbits with about 2^11
   // groups per window (measured at 2^26, window 20: 2^11 groups 157 ms, 2^9
@@ -706,14 +836,22 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       WinDev &wd = wins[w];
       uint32_t log_nb = 0;
       while ((1u << log_nb) < wd.nb) log_nb++;
-      const uint32_t lo = log_nb > 12 ? log_nb - 12 : 0;
-      wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > 11 ? log_nb - 11 : 0));
+      // a shifted-table job's one big window: up to 2^14 groups
+      const uint32_t hb = d.pre ? 14 : 12, hd = d.pre ? 14 : 11;
+      const uint32_t lo = log_nb > hb ? log_nb - hb : 0;
+      wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > hd ? log_nb - hd : 0));
       wd.g0 = (uint32_t)GT;
       const uint32_t H = bucket_groups(wd.nb, wd.lbits);
       GT += H;
       max_groups = std::max(max_groups, H);
       max_lbits = std::max(max_lbits, wd.lbits);
       kj += wd.nb;
+    }
+    // digit-only windows of a shifted-table job: no groups (group lookups
+    // take the last window whose first group is <= g, the next sort window)
+    for (uint32_t w = d.dwin0; d.pre && w < d.dwin0 + d.dW; w++) {
+      wins[w].g0 = (uint32_t)GT;
+      wins[w].lbits = 0;
     }
     counters += (msd_path ? GT - d.g0 : kj) * d.chunks;
   }
@@ -878,8 +1016,21 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, ws.wins,
                      ws.gstart, Wt, log_g, ws.groups, KC);
   ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_window_sum, dim3(Wt), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups,
-                     ws.gstart, ws.winsum, KC);
+  // Windows of many groups (a shifted-table job's one window) are summed in
+  // slices, then the slices per window.
+  uint32_t max_wgroups = 0;
+  for (uint32_t w = 0; w < Wt; w++) max_wgroups = std::max(max_wgroups, gstart[w + 1] - gstart[w]);
+  const uint32_t S = max_wgroups > 8192 ? 32 : 1;
+  if (S > 1) {
+    ZP_TRY(grow(ws.winpart, ws.cap_winpart, (size_t)Wt * S));
+    hipLaunchKernelGGL(k_window_sum, dim3(Wt, S), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups,
+                       ws.gstart, ws.winpart, KC);
+    ZP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_window_slices, dim3(grid_for(Wt, 64)), dim3(64), 0, stream, ws.winpart, Wt, S, ws.winsum, KC);
+  } else {
+    hipLaunchKernelGGL(k_window_sum, dim3(Wt), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups,
+                       ws.gstart, ws.winsum, KC);
+  }
   ZP_HIP(hipGetLastError());
   ws.h_winsum.resize(Wt);
   ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
@@ -1076,6 +1227,102 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
   return ZP_OK;
 }
 
+// Width of the window-shifted table for n points ("precompute=auto"): the
+// cheapest (msm_cost, shared buckets) among the widths whose top window is
+// nearly full -- 253 = 10 x 23 + 23 = 14 x 17 + 15 = 15 x 16 + 13 =
+// 16 x 15 + 13 = 22 x 11 + 11.  A short top window (22 bits: 11 x 22 + 11)
+// puts every point's top digit into the few lowest buckets, i.e. into one or
+// two groups of the bucket sort, and the block sorting them serializes
+// (measured at 2^24: 27 ms against 1.5 ms).  2^26 points: 23 bits (11 tables).
+static int auto_shift_width(size_t n) {
+  static const int widths[] = {23, 17, 16, 15, 11};
+  int best = 11;
+  double best_cost = 0;
+  for (int c : widths) {
+    HostLayout L = layout_uniform(c);
+    layout_finish(L);
+    const double cost = msm_cost(n, L, true);
+    if (best_cost == 0 || cost < best_cost) {
+      best = c;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+// The window-shifted table for the uniform layout of c bits (see k_shift_a).
+// Slices of points bound the temporaries; entry w of a slice is made from
+// its entry w - 1, window after window.
+static int init_shift_table(zp_msm_context *ctx, int c) {
+  const FConsts &K = consts();
+  const size_t n = ctx->n;
+  HostLayout L = layout_uniform(c);
+  const uint32_t W = (uint32_t)L.W;
+  ZP_CHECK((uint64_t)n * W < (1ull << 31), ZP_ERR_TOO_LARGE, "msm_init: precompute: %zu points x %u windows >= 2^31",
+           n, W);
+  if (n == 0) return ZP_OK;
+  size_t free_b = 0, total_b = 0;
+  ZP_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t bytes = n * W * sizeof(PackedPre);
+  ZP_CHECK(bytes + (size_t(4) << 30) < free_b, ZP_ERR_OUT_OF_MEMORY,
+           "msm_init: precompute=%d needs %.1f GB for %u shifted tables, %.1f GB free", c, bytes / 1e9, W,
+           free_b / 1e9);
+  ZP_HIP(hipMalloc(&ctx->d_shift, bytes));
+  const size_t SLICE = std::min<size_t>(n, (size_t)1 << 22);
+  F *X = nullptr, *Y = nullptr, *Z = nullptr, *T = nullptr;
+  int *d_bad = nullptr;
+  int st = ZP_OK;
+  if (hipMalloc(&X, SLICE * sizeof(F)) != hipSuccess || hipMalloc(&Y, SLICE * sizeof(F)) != hipSuccess ||
+      hipMalloc(&Z, SLICE * sizeof(F)) != hipSuccess || hipMalloc(&T, SLICE * sizeof(F)) != hipSuccess ||
+      hipMalloc(&d_bad, sizeof(int)) != hipSuccess) {
+    set_last_error("msm_init: precompute: temporary allocation failed");
+    st = ZP_ERR_OUT_OF_MEMORY;
+  }
+  hipStream_t s = ctx->stream;
+  if (st == ZP_OK) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, s, (uint32_t *)d_bad, (uint64_t)1, 0u);
+    hipLaunchKernelGGL(k_shift_copy0, dim3(grid_for(n)), dim3(MSM_THREADS), 0, s, ctx->d_pts, (uint64_t)n, W,
+                       ctx->d_shift);
+    for (uint32_t w = 1; w < W; w++) {
+      for (size_t b = 0; b < n; b += SLICE) {
+        const size_t m = std::min(SLICE, n - b);
+        hipLaunchKernelGGL(k_shift_a, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, ctx->d_shift, (uint64_t)b,
+                           (uint64_t)m, W, w, (uint32_t)c, ctx->d_conv, K, X, Y, Z, d_bad);
+        const uint32_t chunk = 32;
+        hipLaunchKernelGGL(k_batch_inv, dim3(grid_for((m + chunk - 1) / chunk, 64)), dim3(64), 0, s, Z, (uint64_t)m,
+                           chunk, T, K);
+        hipLaunchKernelGGL(k_shift_c, dim3(grid_for(m)), dim3(MSM_THREADS), 0, s, X, Y, Z, (uint64_t)b, (uint64_t)m,
+                           W, w, K, ctx->d_shift);
+      }
+    }
+    if (hipGetLastError() != hipSuccess) {
+      set_last_error("msm_init: precompute launch failed");
+      st = ZP_ERR_HIP;
+    }
+  }
+  int bad = 0;
+  if (st == ZP_OK && (hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                      hipStreamSynchronize(s) != hipSuccess)) {
+    set_last_error("msm_init: precompute failed");
+    st = ZP_ERR_HIP;
+  }
+  if (X) (void)hipFree(X);
+  if (Y) (void)hipFree(Y);
+  if (Z) (void)hipFree(Z);
+  if (T) (void)hipFree(T);
+  if (d_bad) (void)hipFree(d_bad);
+  if (st != ZP_OK || bad) {
+    // A table point of small order has a multiple without a twisted-Edwards
+    // image: jobs on this table keep the per-window buckets.
+    (void)hipFree(ctx->d_shift);
+    ctx->d_shift = nullptr;
+    return st;
+  }
+  ctx->shift_c = c;
+  ctx->shift_W = (int)W;
+  return ZP_OK;
+}
+
 static int msm_init_common(const char *options, size_t options_len, const g1_affine_t *points, size_t npoints,
                            bool on_device, zp_msm_context **out) {
   ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "msm_init: out is NULL");
@@ -1098,6 +1345,8 @@ static int msm_init_common(const char *options, size_t options_len, const g1_aff
     st = ZP_ERR_HIP;
   }
   if (st == ZP_OK) st = init_points(ctx, points, on_device);
+  if (st == ZP_OK && opts.precompute)
+    st = init_shift_table(ctx, opts.precompute > 0 ? opts.precompute : auto_shift_width(npoints));
   if (st != ZP_OK) {
     msm_free(ctx);
     return st;
@@ -1138,6 +1387,7 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   if (ctx->d_scalars2) (void)hipFree(ctx->d_scalars2);
   ctx->uploader.release();
   if (ctx->d_pts) (void)hipFree(ctx->d_pts);
+  if (ctx->d_shift) (void)hipFree(ctx->d_shift);
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
   if (ctx->d_nc_idx) (void)hipFree(ctx->d_nc_idx);
   if (ctx->d_nc_off) (void)hipFree(ctx->d_nc_off);

@@ -45,6 +45,13 @@ struct MsmJobDev {
   uint64_t cnt0;            // first per-(bucket, chunk) counter: cnt0 + b_local * C + c
                             // (two-level path: per-(group, chunk), cnt0 + (g - g0) * C + c)
   uint32_t g0;              // two-level path: global index of the job's first bucket group
+  // Digit generation: windows [dwin0, dwin0 + dW) of the scalar layout over
+  // the dn real points.  Equal to (win0, W, n) except for a job on a
+  // window-shifted table (pre = 1): there the dW digits of point i are the
+  // entries i * dW + w of ONE sort window over n = dn * dW virtual points, and
+  // virtual point i * dW + w is table entry 2^(off_w) P_i.
+  uint32_t dwin0, dW, dn;
+  uint32_t pre;
 };
 
 // One block of the sorted path: window w (global) of job j, chunk c.
@@ -158,19 +165,39 @@ __global__ void __launch_bounds__(MSM_THREADS) k_digits(BatchDev B, uint32_t *cn
 // Signed digits of every (job, point, window), zero for points without a
 // twisted-Edwards image (their scalars are handled by the host fallback).
 __global__ void __launch_bounds__(MSM_THREADS) k_digits_make(BatchDev B, int32_t *digits) {
-  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < B.total_points; g += gridDim.x * blockDim.x) {
-    const uint32_t j = upper_index(B.job_pt0, B.J, g);
-    const MsmJobDev jb = B.jobs[j];
-    const uint32_t i = g - jb.pt0;
-    int32_t *dst = digits + jb.dig0 + i;
-    if (!jb.conv[i]) {
-      for (uint32_t w = 0; w < jb.W; w++) dst[(uint64_t)w * jb.n] = 0;
-      continue;
+  // A block whose points all belong to one shifted-table job writes their
+  // digits (point-major there) through LDS, as one contiguous run.
+  __shared__ int32_t stage[MSM_THREADS * 32];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < B.total_points; base += gridDim.x * blockDim.x) {
+    const uint32_t last = min(B.total_points, base + blockDim.x) - 1;
+    const uint32_t jf = upper_index(B.job_pt0, B.J, base);
+    const bool staged = jf == upper_index(B.job_pt0, B.J, last) && B.jobs[jf].pre;  // block-uniform
+    const uint32_t g = base + threadIdx.x;
+    if (g <= last) {
+      const uint32_t j = upper_index(B.job_pt0, B.J, g);
+      const MsmJobDev jb = B.jobs[j];
+      const uint32_t i = g - jb.pt0;
+      // digit (w, i) at dig0 + w * n + i; on a window-shifted table at dig0 + i * dW + w
+      int32_t *dst = staged ? stage + threadIdx.x * jb.dW
+                            : digits + jb.dig0 + (jb.pre ? (uint64_t)i * jb.dW : (uint64_t)i);
+      const uint64_t step = jb.pre ? 1 : jb.dn;
+      if (!jb.conv[i]) {
+        for (uint32_t w = 0; w < jb.dW; w++) dst[w * step] = 0;
+      } else {
+        uint64_t s[4];
+        load_scalar(jb.scalars, i, s);
+        uint32_t carry = 0;
+        for (uint32_t w = 0; w < jb.dW; w++) dst[w * step] = window_digit(s, B.wins[jb.dwin0 + w], carry);
+      }
     }
-    uint64_t s[4];
-    load_scalar(jb.scalars, i, s);
-    uint32_t carry = 0;
-    for (uint32_t w = 0; w < jb.W; w++) dst[(uint64_t)w * jb.n] = window_digit(s, B.wins[jb.win0 + w], carry);
+    if (staged) {
+      __syncthreads();
+      const MsmJobDev jb = B.jobs[jf];
+      const uint32_t cnt = (last - base + 1) * jb.dW;
+      int32_t *dst = digits + jb.dig0 + (uint64_t)(base - jb.pt0) * jb.dW;
+      for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) dst[k] = stage[k];
+      __syncthreads();
+    }
   }
 }
 
@@ -492,12 +519,17 @@ __global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *bucke
   }
 }
 
-// One workgroup per window: tree sum of its groups into out[w].
+// Workgroup (w, s) of a (windows x S) grid: tree sum of slice s of window
+// w's groups into out[w * S + s] (S = 1: the window's sum).  A window of
+// 2^23 buckets has ~2^17 groups; with S slices no thread adds more than a
+// few hundred of them in sequence.
 __global__ void __launch_bounds__(MSM_THREADS) k_window_sum(const Xyzt *groups, const uint32_t *gstart, Xyzt *out,
                                                             FConsts K) {
   extern __shared__ Xyzt sh[];
-  const uint32_t w = blockIdx.x;
-  const uint32_t g0 = gstart[w], g1 = gstart[w + 1];
+  const uint32_t w = blockIdx.x, S = gridDim.y, sl = blockIdx.y;
+  const uint32_t a = gstart[w], b = gstart[w + 1];
+  const uint32_t per = (b - a + S - 1) / S;
+  const uint32_t g0 = min(b, a + sl * per), g1 = min(b, g0 + per);
   Xyzt acc = ec_identity(K);
   for (uint32_t g = g0 + threadIdx.x; g < g1; g += blockDim.x) ec_add(acc, groups[g], K);
   sh[threadIdx.x] = acc;
@@ -510,7 +542,16 @@ __global__ void __launch_bounds__(MSM_THREADS) k_window_sum(const Xyzt *groups, 
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[w] = sh[0];
+  if (threadIdx.x == 0) out[(uint64_t)w * S + sl] = sh[0];
+}
+
+// out[w] = sum of the S slice sums part[w * S .. w * S + S) (one thread per window).
+__global__ void k_window_slices(const Xyzt *part, uint32_t W_total, uint32_t S, Xyzt *out, FConsts K) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W_total; w += gridDim.x * blockDim.x) {
+    Xyzt acc = part[(uint64_t)w * S];
+    for (uint32_t s = 1; s < S; s++) ec_add(acc, part[(uint64_t)w * S + s], K);
+    out[w] = acc;
+  }
 }
 
 }  // namespace zp
