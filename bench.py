@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--msm-log-n", type=int, default=26)
     ap.add_argument("--msm-steps", type=int, default=3)
     ap.add_argument("--msm-warmup", type=int, default=1)
+    ap.add_argument("--batch-options", default="",
+                    help="msm_init option string of the batched form's shared 2^22 table")
     ap.add_argument("--msm-options", default="precompute=auto",
                     help="msm_init option string of the 2^N table ('' = default options; otherwise the "
                          "default-option table is timed too, as plain_table)")
@@ -281,7 +283,7 @@ def bench_batched(args):
     from hardcaml_zprize_amd._lib import call
 
     rng = np.random.default_rng(args.seed + 2000)
-    ctx, _, _ = make_table(zmsm, rng, 22)
+    ctx, _, _ = make_table(zmsm, rng, 22, args.batch_options)
     P = 2**64 - 2**32 + 1
     ntt_logs = [int(x) for x in rng.integers(10, 21, size=256)]
     ntt_dirs = [int(x) for x in rng.integers(0, 2, size=256)]

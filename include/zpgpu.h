@@ -143,8 +143,14 @@ typedef struct zp_msm_context zp_msm_context;
  * it may be NULL/empty.  "coords=canonical" takes x and y as plain integers
  * < q (the form ark-serialize files hold) instead of x R, y R; the default
  * is "coords=montgomery".  Points are converted on the GPU to the
- * twisted-Edwards precomputed form and stay resident in HBM.  Returns NULL
- * on failure (zp_last_error() says why). */
+ * twisted-Edwards precomputed form and stay resident in HBM.
+ * "precompute=<c>" (c in [8, 24]) or "precompute=auto" also keeps the
+ * window-shifted copies 2^(c w) P_i for every window w of the uniform c-bit
+ * layout (ceil(253 / c) copies, 144 bytes each: 106 GB for 2^26 points at
+ * the automatic c = 23); an MSM over the table, or over a prefix large enough
+ * for it to pay, then adds every window's digits into one bucket set.  Fails
+ * with ZP_ERR_OUT_OF_MEMORY when the copies do not fit.  Results are the same
+ * bytes either way.  Returns NULL on failure (zp_last_error() says why). */
 zp_msm_context *msm_init(const char *xclbin, ssize_t xclbin_len, g1_affine_t *rust_points, ssize_t npoints);
 
 /* extern "C" void msm_mult(Driver *driver, g1_projective_t *out,
