@@ -32,8 +32,8 @@ def parse():
     ap.add_argument("--msm-log-n", type=int, default=26)
     ap.add_argument("--msm-steps", type=int, default=3)
     ap.add_argument("--msm-warmup", type=int, default=1)
-    ap.add_argument("--batch-options", default="",
-                    help="msm_init option string of the batched form's shared 2^22 table")
+    ap.add_argument("--batch-options", default="precompute=auto",
+                    help="msm_init option string of the batched form's shared 2^22 table (both forms use it)")
     ap.add_argument("--msm-options", default="precompute=auto",
                     help="msm_init option string of the 2^N table ('' = default options; otherwise the "
                          "default-option table is timed too, as plain_table)")

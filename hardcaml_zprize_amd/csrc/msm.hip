@@ -836,8 +836,11 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       WinDev &wd = wins[w];
       uint32_t log_nb = 0;
       while ((1u << log_nb) < wd.nb) log_nb++;
-      // a shifted-table job's one big window: up to 2^14 groups
-      const uint32_t hb = d.pre ? 14 : 12, hd = d.pre ? 14 : 11;
+      // a shifted-table job's one window: about 2^16 entries per group,
+      // 2^11 to 2^14 groups (pass B's scattered writes stay within L2)
+      uint32_t hp = 11;
+      while (d.pre && hp < 14 && ((uint64_t)d.n >> 16) > (1ull << hp)) hp++;
+      const uint32_t hb = d.pre ? hp : 12, hd = d.pre ? hp : 11;
       const uint32_t lo = log_nb > hb ? log_nb - hb : 0;
       wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > hd ? log_nb - hd : 0));
       wd.g0 = (uint32_t)GT;
