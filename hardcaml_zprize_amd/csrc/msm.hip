@@ -412,7 +412,7 @@ struct zp_msm_workspace {
   size_t cap_jobs = 0, cap_pt0 = 0, cap_b0 = 0, cap_scan = 0;
   size_t cap_digits = 0, cap_counts = 0, cap_scanned = 0, cap_segs = 0, cap_tmp = 0, cap_wg0 = 0;
   int32_t *digits = nullptr;
-  uint2 *tmp = nullptr;         // two-level sort: pass A output
+  uint32_t *tmp = nullptr;      // two-level sort: pass A output
   uint32_t *win_g0 = nullptr;   // two-level sort: first group of each window
   uint32_t *counts = nullptr, *scanned = nullptr;
   SegDev *segs = nullptr;
@@ -800,7 +800,9 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       // 2^16 points (up to 256 of them) keep each scatter block's output small
       // enough for L2 to merge its partial-line writes (measured at 2^26:
       // 64 chunks 179 ms, 128 174, 256 170-175, 512 174).
-      d.chunks = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, h.n >> 16));
+      // (beyond 2^26 points, chunks of 2^18: pass A stores a chunk offset in
+      // 31 - lbits bits)
+      d.chunks = (uint32_t)std::max<uint64_t>(1, std::max<uint64_t>(std::min<uint64_t>(256, h.n >> 16), h.n >> 18));
     }
     d.chunk_len = (uint32_t)std::max<uint64_t>(1, ((uint64_t)d.n + d.chunks - 1) / d.chunks);
     d.dig0 = entries;
@@ -826,7 +828,8 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   // (pass A's LDS histogram); a few-bucket top window gets one bucket per group.
   const int lb_opt = bopts.sort_lbits >= 0 ? bopts.sort_lbits : jobs[0].ctx->opts.sort_lbits;
   uint64_t GT = 0;
-  uint32_t max_lbits = 0, max_groups = 1;
+  uint32_t max_lbits = 0, max_groups = 1, max_chunks = 1;
+  for (size_t j = 0; j < count; j++) max_chunks = std::max(max_chunks, jd[j].chunks);
   for (size_t j = 0; j < count; j++) {
     MsmJobDev &d = jd[j];
     d.cnt0 = counters;
@@ -844,6 +847,8 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       const uint32_t lo = log_nb > hb ? log_nb - hb : 0;
       wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > hd ? log_nb - hd : 0));
       wd.g0 = (uint32_t)GT;
+      ZP_CHECK(wd.lbits == 0 || ((uint64_t)d.chunk_len << (wd.lbits + 1)) <= (1ull << 32), ZP_ERR_TOO_LARGE,
+               "MSM batch: chunk of %u entries with %u low bucket bits", d.chunk_len, wd.lbits);
       const uint32_t H = bucket_groups(wd.nb, wd.lbits);
       GT += H;
       max_groups = std::max(max_groups, H);
@@ -954,7 +959,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
     }
   } else if (msd_path) {
     const size_t lds_a = (size_t)max_groups * sizeof(uint32_t);
-    const size_t lds_b = ((2u << max_lbits) + 1024) * sizeof(uint32_t);
+    const size_t lds_b = ((2u << max_lbits) + 1024 + max_chunks + 1) * sizeof(uint32_t);
     if (points) {
       hipLaunchKernelGGL(k_digits_make, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.digits);
       ZP_HIP(hipGetLastError());

@@ -297,12 +297,14 @@ __global__ void __launch_bounds__(1024) k_msd_count(BatchDev B, const SegDev *se
   for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) dst[(uint64_t)h * jb.chunks] = hist[h];
 }
 
-// Pass A scatter: tmp[pos] = (point | sign << 31, local bucket), grouped by
-// (window, group), chunks in order within a group.  A window whose groups are
-// single buckets (lbits = 0) is sorted by now: its entries go straight to
-// vals (same slots), and pass B only writes their offsets.
+// Pass A scatter: tmp[pos] = (i - chunk start) << (lbits + 1) | low lbits of
+// the bucket << 1 | sign, grouped by (window, group), chunks in order within
+// a group (pass B recovers the chunk from the group's chunk offsets; the
+// host keeps chunk_len << (lbits + 1) within 32 bits).  A window whose groups
+// are single buckets (lbits = 0) is sorted by now: its entries go straight to
+// vals (same slots, point | sign << 31), and pass B only writes their offsets.
 __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *segs, const int32_t *digits,
-                                                      const uint32_t *scanned, uint2 *tmp, uint32_t *vals) {
+                                                      const uint32_t *scanned, uint32_t *tmp, uint32_t *vals) {
   extern __shared__ uint32_t cursor[];
   const SegDev sg = segs[blockIdx.x];
   const MsmJobDev jb = B.jobs[sg.job];
@@ -319,20 +321,21 @@ __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *
     if (d == 0) continue;
     const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
     const uint32_t pos = lds_claim(cursor, b >> lbits);
-    const uint32_t v = i | (d < 0 ? 0x80000000u : 0u);
+    const uint32_t sign = d < 0 ? 1u : 0u;
     if (lbits == 0)
-      vals[pos] = v;
+      vals[pos] = i | sign << 31;
     else
-      tmp[pos] = make_uint2(v, b);
+      tmp[pos] = (i - i0) << (lbits + 1) | (b & ((1u << lbits) - 1)) << 1 | sign;
   }
 }
 
 // Pass B: block g sorts bucket group g (window w, high part h) by the low
 // lbits of its window: histogram, exclusive scan, off/cnt of the group's
-// buckets, then placement into vals.  The last group also writes off[K] (the
-// total).  LDS: 2 * 2^max_lbits + 1024 words.
+// buckets, then placement into vals (point | sign << 31, the point being its
+// chunk's first point plus the offset pass A stored).  The last group also
+// writes off[K] (the total).  LDS: 2 * 2^max_lbits + 1024 + max_chunks + 1 words.
 __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *win_g0, uint32_t Wt, uint32_t GT,
-                                                    uint32_t max_lbits, const uint32_t *scanned, const uint2 *tmp,
+                                                    uint32_t max_lbits, const uint32_t *scanned, const uint32_t *tmp,
                                                     uint32_t *vals, uint32_t *off, uint32_t *cnt) {
   extern __shared__ uint32_t lds[];
   const uint32_t g = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
@@ -344,6 +347,7 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
   const uint32_t b0 = (g - wd.g0) << wd.lbits;
   const uint32_t nl = min(1u << wd.lbits, wd.nb - b0);  // buckets in this group
   uint32_t *hist = lds, *cur = lds + (1u << max_lbits), *part = cur + (1u << max_lbits);
+  uint32_t *cs = part + nt;  // the group's chunk offsets: chunk c's entries are [cs[c], cs[c + 1])
   if (wd.lbits == 0) {  // one bucket, placed by pass A
     if (tid == 0) {
       off[wd.base + b0] = start;
@@ -352,9 +356,11 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
     }
     return;
   }
+  const uint32_t lmask = (1u << wd.lbits) - 1;
   for (uint32_t l = tid; l < nl; l += nt) hist[l] = 0;
+  for (uint32_t c = tid; c <= jb.chunks; c += nt) cs[c] = scanned[ci + c];
   __syncthreads();
-  for (uint32_t p = start + tid; p < end; p += nt) lds_claim(hist, tmp[p].y - b0);
+  for (uint32_t p = start + tid; p < end; p += nt) lds_claim(hist, (tmp[p] >> 1) & lmask);
   __syncthreads();
   // exclusive scan of hist[0, nl): each thread sums a run of `per`, a
   // Hillis-Steele scan over the 1024 run sums, then each run is expanded.
@@ -381,8 +387,17 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
   if (g == GT - 1 && tid == 0) off[B.K] = end;
   __syncthreads();
   for (uint32_t p = start + tid; p < end; p += nt) {
-    const uint2 e = tmp[p];
-    vals[lds_claim(cur, e.y - b0)] = e.x;
+    const uint32_t e = tmp[p];
+    uint32_t lo = 0, hi = jb.chunks;  // cs[lo] <= p < cs[hi]: find p's chunk
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (cs[mid] <= p)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t i = lo * jb.chunk_len + (e >> (wd.lbits + 1));
+    vals[lds_claim(cur, (e >> 1) & lmask)] = i | (e & 1u) << 31;
   }
 }
 
