@@ -1272,9 +1272,13 @@ static int init_shift_table(zp_msm_context *ctx, int c) {
   size_t free_b = 0, total_b = 0;
   ZP_HIP(hipMemGetInfo(&free_b, &total_b));
   const size_t bytes = n * W * sizeof(PackedPre);
-  ZP_CHECK(bytes + (size_t(4) << 30) < free_b, ZP_ERR_OUT_OF_MEMORY,
-           "msm_init: precompute=%d needs %.1f GB for %u shifted tables, %.1f GB free", c, bytes / 1e9, W,
-           free_b / 1e9);
+  // and room for one full-table MSM's workspace: per (point, window) a digit,
+  // a pass-A entry and a sorted entry, the unit partials and the shared
+  // buckets, plus slack
+  const size_t ws_bytes = n * W * 16 + (size_t(1) << (c - 1)) * 3 * sizeof(Xyzt) + (size_t(2) << 30);
+  ZP_CHECK(bytes + ws_bytes < free_b, ZP_ERR_OUT_OF_MEMORY,
+           "msm_init: precompute=%d needs %.1f GB for %u shifted tables (+%.1f GB of MSM workspace), %.1f GB free",
+           c, bytes / 1e9, W, ws_bytes / 1e9, free_b / 1e9);
   ZP_HIP(hipMalloc(&ctx->d_shift, bytes));
   const size_t SLICE = std::min<size_t>(n, (size_t)1 << 22);
   F *X = nullptr, *Y = nullptr, *Z = nullptr, *T = nullptr;
