@@ -40,9 +40,9 @@ def parse():
     ap.add_argument("--msm-options", default="precompute=auto",
                     help="msm_init option string of the 2^N table ('' = default options; otherwise the "
                          "default-option table is timed too, as plain_table)")
-    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-jobs", type=int, default=240)
     ap.add_argument("--serve-load", type=float, default=6.0)
-    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--serve-batch-ms", type=float, default=100.0,
                     help="estimated work per served batch (ms); 0 = every waiting job that fits the budget")
     ap.add_argument("--serve-load2", type=float, default=0.7,
