@@ -161,7 +161,7 @@ static int launch_4096_cols(const NttTables *t, uint64_t *d_in, uint64_t *d_out,
   a.out = d_out;
   a.in_tab = jt.data;
   a.out_tab = jt.scratch;
-  a.scale = 1;
+  a.scale = dir ? t->n_inv : 1;  // applied with the 4-step twiddle; pass 2 does not scale
   const dim3 grid(a.tiles, jt.jobs);
   if (dir)
     hipLaunchKernelGGL((k_ntt4096_cols<T, 1>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
@@ -183,7 +183,7 @@ static int launch_4096_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out,
   a.out = d_out;
   a.in_tab = jt.scratch;
   a.out_tab = jt.data;
-  a.scale = dir ? t->n_inv : 1;
+  a.scale = 1;  // the inverse's n^-1 is applied in pass 1
   const dim3 grid(a.tiles, jt.jobs);
   if (dir)
     hipLaunchKernelGGL((k_ntt4096_rows<T, 1>), grid, dim3(64 * T), ntt4096_lds_bytes<T>(), stream, a);
@@ -329,7 +329,8 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
   p2.out_elem_stride = n1;
   p2.out_seq_stride = 1;
   p2.tw_sub = t->d_sub[dir];
-  p2.scale = scale;
+  // after the register-resident 4096-point columns, n^-1 is already applied
+  p2.scale = (sh.log_n1 == 12 && sh.log_n2 >= 3) ? 1 : scale;
   p2.in_tab = jt.scratch;
   p2.out_tab = jt.data;
   return launch_dft(p2, n1 >> p2.log_T, stream, jt.jobs);

@@ -220,7 +220,7 @@ struct Ntt4096Args {
   const uint64_t *tw4096;  // [k][a] = w_4096^(a k), a, k < 64 (direction-specific)
   const uint64_t *tw_lo;   // w_n^i, i < 4096 (pass 1 only)
   const uint64_t *tw_hi;   // w_n^(4096 i)
-  uint64_t scale;          // pass 2 of the inverse: n^-1 (unused otherwise)
+  uint64_t scale;          // pass 1 of the inverse: n^-1 (unused otherwise)
   uint32_t tiles;          // number of T-wide tiles
   // Batched form: when set, job blockIdx.y reads in_tab[y] and writes out_tab[y].
   const uint64_t *const *in_tab;
@@ -311,10 +311,12 @@ k_ntt4096_cols(Ntt4096Args args) {
   exchange<T, false>(x, lds, s, a, s, a);
   dft64_regs<DIR, true>(x);
   // x[k''] = output k1 = a + 64 k'' (a now plays k').  4-step twiddle
-  // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''.  The scratch matrix
-  // holds lazy (not necessarily canonical) values; pass 2 canonicalizes the
-  // ones its first butterfly stage needs.
+  // w_n^(col * k1) = w_n^(col * a) * (w_n^(64 col))^k''; the inverse's n^-1
+  // rides on its base (one multiply per lane, none per element).  The scratch
+  // matrix holds lazy (not necessarily canonical) values; pass 2
+  // canonicalizes the ones its first butterfly stage needs.
   uint64_t w = gl_mul(th0, tl0);
+  if constexpr (DIR == 1) w = gl_mul(w, args.scale);
   const uint64_t step = gl_mul(th1, tl1);
 #pragma unroll
   for (int k = 0; k < 64; k++) {
@@ -355,9 +357,7 @@ k_ntt4096_rows(Ntt4096Args args) {
   const uint32_t lane_out = kp * (uint32_t)n1 + r2;
 #pragma unroll
   for (int k = 0; k < 64; k++) {
-    uint64_t v = x[k];
-    if constexpr (DIR == 1) v = gl_mul_lz(v, args.scale);  // n^-1 (the launcher sets it for DIR 1 only)
-    v = gl_canon(v);
+    const uint64_t v = gl_canon(x[k]);  // n^-1 of the inverse was applied in pass 1
     st_u64(args.out + (uint64_t)(64 * k) * n1 + row0, lane_out, v);
   }
 }
