@@ -1021,7 +1021,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.nunits,
                      (uint32_t)K, ws.buckets, KC);
   ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, ws.wins,
+  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, ws.cnt, ws.wins,
                      ws.gstart, Wt, log_g, ws.groups, KC);
   ZP_HIP(hipGetLastError());
   // Windows of many groups (a shifted-table job's one window) are summed in

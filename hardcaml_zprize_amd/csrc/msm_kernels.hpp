@@ -510,23 +510,28 @@ __device__ __forceinline__ Xyzt ec_mul_small(const Xyzt &p, uint32_t m, const FC
 }
 
 // Group j of window w covers buckets [j g, j g + g) clipped to nb[w];
-// gstart[w] = first group of window w (W_total + 1 entries).
-__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *buckets, const WinDev *wins,
-                                                               const uint32_t *gstart, uint32_t W_total,
-                                                               uint32_t log_g, Xyzt *groups, FConsts K) {
+// gstart[w] = first group of window w (W_total + 1 entries).  Empty buckets
+// (cnt = 0) above a group's highest non-empty one are skipped: the running
+// sum is still the identity there (a shifted-table job's shared buckets above
+// the largest digit are all empty).
+__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *buckets, const uint32_t *cnt,
+                                                               const WinDev *wins, const uint32_t *gstart,
+                                                               uint32_t W_total, uint32_t log_g, Xyzt *groups,
+                                                               FConsts K) {
   const uint32_t total = gstart[W_total];
   for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < total; gi += gridDim.x * blockDim.x) {
     const uint32_t w = upper_index(gstart, W_total, gi);
     const WinDev wd = wins[w];
     const uint32_t lo = (gi - gstart[w]) << log_g;
-    const uint32_t hi = min(wd.nb, lo + (1u << log_g));
+    uint32_t hi = min(wd.nb, lo + (1u << log_g));
+    while (hi > lo && cnt[wd.base + hi - 1] == 0) hi--;
     Xyzt run = ec_identity(K), acc = ec_identity(K);
     for (int b = (int)hi - 1; b >= (int)lo; b--) {
       ec_add(run, buckets[wd.base + b], K);
       ec_add(acc, run, K);
     }
     // bucket b holds digit magnitude b + 1: weight (b + 1) = (b - lo + 1) + lo
-    if (lo) {
+    if (lo && hi > lo) {
       const Xyzt t = ec_mul_small(run, lo, K);
       ec_add(acc, t, K);
     }
