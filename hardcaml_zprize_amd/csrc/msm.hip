@@ -897,7 +897,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
                                  stream));
   if (msd_path) {
-    ZP_TRY(grow(ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1)));
+    ZP_TRY(grow(ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1) * (any_pre ? 1 : 2)));
     ZP_TRY(grow(ws.win_g0, ws.cap_wg0, std::max<size_t>(wins.size(), 1)));
   }
   if (sorted_path || msd_path) {
@@ -972,12 +972,20 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
     ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb0, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
                                    rocprim::plus<uint32_t>(), stream));
     if (points) {
-      hipLaunchKernelGGL(k_msd_scatter, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
-                         ws.digits, ws.scanned, ws.tmp, ws.vals);
+      if (any_pre)
+        hipLaunchKernelGGL(k_msd_scatter<true>, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
+                           ws.digits, ws.scanned, ws.tmp, ws.vals);
+      else
+        hipLaunchKernelGGL(k_msd_scatter<false>, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
+                           ws.digits, ws.scanned, ws.tmp, ws.vals);
       ZP_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_msd_group, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt, (uint32_t)GT,
-                       max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
+    if (any_pre)
+      hipLaunchKernelGGL(k_msd_group<true>, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt,
+                         (uint32_t)GT, max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
+    else
+      hipLaunchKernelGGL(k_msd_group<false>, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt,
+                         (uint32_t)GT, max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
     ZP_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);

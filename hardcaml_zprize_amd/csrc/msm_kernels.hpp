@@ -297,12 +297,16 @@ __global__ void __launch_bounds__(1024) k_msd_count(BatchDev B, const SegDev *se
   for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) dst[(uint64_t)h * jb.chunks] = hist[h];
 }
 
-// Pass A scatter: tmp[pos] = (i - chunk start) << (lbits + 1) | low lbits of
-// the bucket << 1 | sign, grouped by (window, group), chunks in order within
-// a group (pass B recovers the chunk from the group's chunk offsets; the
-// host keeps chunk_len << (lbits + 1) within 32 bits).  A window whose groups
-// are single buckets (lbits = 0) is sorted by now: its entries go straight to
-// vals (same slots, point | sign << 31), and pass B only writes their offsets.
+// Pass A scatter, grouped by (window, group), chunks in order within a
+// group.  COMPACT (batches with a shifted-table job): 4-byte entries
+// tmp[pos] = (i - chunk start) << (lbits + 1) | low lbits of the bucket << 1
+// | sign (pass B recovers the chunk from the group's chunk offsets; the host
+// keeps chunk_len << (lbits + 1) within 32 bits).  Otherwise 8-byte entries
+// (point | sign << 31, bucket), which the per-window buckets' scatter of 2^11
+// runs per chunk writes faster.  A window whose groups are single buckets
+// (lbits = 0) is sorted by now: its entries go straight to vals (same slots,
+// point | sign << 31), and pass B only writes their offsets.
+template <bool COMPACT>
 __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *segs, const int32_t *digits,
                                                       const uint32_t *scanned, uint32_t *tmp, uint32_t *vals) {
   extern __shared__ uint32_t cursor[];
@@ -324,8 +328,10 @@ __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *
     const uint32_t sign = d < 0 ? 1u : 0u;
     if (lbits == 0)
       vals[pos] = i | sign << 31;
-    else
+    else if (COMPACT)
       tmp[pos] = (i - i0) << (lbits + 1) | (b & ((1u << lbits) - 1)) << 1 | sign;
+    else
+      reinterpret_cast<uint2 *>(tmp)[pos] = make_uint2(i | sign << 31, b);
   }
 }
 
@@ -334,6 +340,7 @@ __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *
 // buckets, then placement into vals (point | sign << 31, the point being its
 // chunk's first point plus the offset pass A stored).  The last group also
 // writes off[K] (the total).  LDS: 2 * 2^max_lbits + 1024 + max_chunks + 1 words.
+template <bool COMPACT>
 __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *win_g0, uint32_t Wt, uint32_t GT,
                                                     uint32_t max_lbits, const uint32_t *scanned, const uint32_t *tmp,
                                                     uint32_t *vals, uint32_t *off, uint32_t *cnt) {
@@ -360,7 +367,9 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
   for (uint32_t l = tid; l < nl; l += nt) hist[l] = 0;
   for (uint32_t c = tid; c <= jb.chunks; c += nt) cs[c] = scanned[ci + c];
   __syncthreads();
-  for (uint32_t p = start + tid; p < end; p += nt) lds_claim(hist, (tmp[p] >> 1) & lmask);
+  const uint2 *tmp2 = reinterpret_cast<const uint2 *>(tmp);
+  for (uint32_t p = start + tid; p < end; p += nt)
+    lds_claim(hist, COMPACT ? (tmp[p] >> 1) & lmask : tmp2[p].y & lmask);
   __syncthreads();
   // exclusive scan of hist[0, nl): each thread sums a run of `per`, a
   // Hillis-Steele scan over the 1024 run sums, then each run is expanded.
@@ -386,7 +395,11 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
   }
   if (g == GT - 1 && tid == 0) off[B.K] = end;
   __syncthreads();
-  for (uint32_t p = start + tid; p < end; p += nt) {
+  for (uint32_t p = start + tid; p < end && !COMPACT; p += nt) {
+    const uint2 e = tmp2[p];
+    vals[lds_claim(cur, e.y & lmask)] = e.x;
+  }
+  for (uint32_t p = start + tid; p < end && COMPACT; p += nt) {
     const uint32_t e = tmp[p];
     uint32_t lo = 0, hi = jb.chunks;  // cs[lo] <= p < cs[hi]: find p's chunk
     while (hi - lo > 1) {
