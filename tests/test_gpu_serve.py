@@ -14,8 +14,11 @@ pytestmark = pytest.mark.gpu
 R = oracle.R_ORDER
 
 
-def test_served_mix_equals_alone_and_oracle():
-    print("serve mix", flush=True)
+@pytest.mark.parametrize("table_options", ["", "precompute=auto"])
+def test_served_mix_equals_alone_and_oracle(table_options):
+    """Also over tables loaded with their window-shifted copies, where the
+    MSM jobs take the one shared bucket set."""
+    print(f"serve mix [{table_options}]", flush=True)
     sys.stdout.flush()
     rng = random.Random(5)
     nrng = np.random.default_rng(5)
@@ -23,7 +26,8 @@ def test_served_mix_equals_alone_and_oracle():
     for t in range(2):
         ks = [rng.randrange(1, R) for _ in range(2048)]
         ks_all.append(ks)
-        tables.append(zmsm.MsmContext(zmsm.generate_points(np.array([oracle.int_to_words(k, 4) for k in ks]))))
+        pts = zmsm.generate_points(np.array([oracle.int_to_words(k, 4) for k in ks]))
+        tables.append(zmsm.MsmContext(pts, table_options))
     specs = []
     for i in range(40):
         if i % 3:
