@@ -421,6 +421,8 @@ struct zp_msm_workspace {
   Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
   Xyzt *winpart = nullptr;  // slice sums of windows with many groups
   size_t cap_winpart = 0;
+  uint32_t *umax = nullptr;  // largest unit count of any bucket (k_units_per_bucket)
+  size_t cap_umax = 0;
   MsmJobDev *jobs = nullptr;
   WinDev *wins = nullptr;
   uint8_t *scan_tmp = nullptr;
@@ -432,7 +434,7 @@ struct zp_msm_workspace {
   void release() {
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
                   groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
-                  winpart};
+                  winpart, umax};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
@@ -1005,8 +1007,10 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       ZP_HIP(hipGetLastError());
     }
   }
+  ZP_TRY(grow(ws.umax, ws.cap_umax, 1));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, stream, ws.umax, (uint64_t)1, 0u);
   hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint32_t)K,
-                     log_u, ws.nunits);
+                     log_u, ws.nunits, ws.umax);
   ZP_HIP(hipGetLastError());
   size_t tb = ws.cap_scan;
   ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, (size_t)K + 1,
@@ -1023,7 +1027,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   const uint64_t units_per_bucket = (n_max + (1ull << log_u) - 1) >> log_u;
   for (uint64_t stride = 1; stride < units_per_bucket; stride <<= 1) {
     hipLaunchKernelGGL(k_unit_tree, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ubucket,
-                       ws.ustart, ws.nunits, total_units, (uint32_t)stride, KC);
+                       ws.ustart, ws.nunits, total_units, ws.umax, (uint32_t)stride, KC);
     ZP_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.nunits,

@@ -432,9 +432,20 @@ __global__ void k_bucket_counts_from_off(const uint32_t *off, uint32_t K, uint32
 }
 
 // nunits[k] = ceil(cnt[k] / U); k == K gets 0 (so the scan's last entry is the total).
-__global__ void k_units_per_bucket(const uint32_t *cnt, uint32_t K, uint32_t log_u, uint32_t *nunits) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= K; k += gridDim.x * blockDim.x)
-    nunits[k] = k < K ? (cnt[k] + (1u << log_u) - 1) >> log_u : 0;
+// *umax (zeroed by the caller) becomes the largest unit count of any bucket
+// when that exceeds 1 (a wave maximum, then one atomic per wave).
+__global__ void k_units_per_bucket(const uint32_t *cnt, uint32_t K, uint32_t log_u, uint32_t *nunits,
+                                   uint32_t *umax) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base <= K; base += gridDim.x * blockDim.x) {  // block-uniform
+    const uint32_t k = base + threadIdx.x;
+    uint32_t v = 0;
+    if (k <= K) {
+      v = k < K ? (cnt[k] + (1u << log_u) - 1) >> log_u : 0;
+      nunits[k] = v;
+    }
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    if ((threadIdx.x & 63) == 0 && v > 1) atomicMax(umax, v);
+  }
 }
 
 // unit -> bucket: the bucket k with ustart[k] <= u < ustart[k + 1] (ustart is
@@ -487,8 +498,12 @@ __global__ void __launch_bounds__(MSM_THREADS) k_accumulate(BatchDev B, const ui
 // unit is both read and written.
 __global__ void __launch_bounds__(MSM_THREADS) k_unit_tree(Xyzt *partial, const uint32_t *ubucket,
                                                            const uint32_t *ustart, const uint32_t *nunits,
-                                                           const uint32_t *total_units, uint32_t stride,
-                                                           FConsts K) {
+                                                           const uint32_t *total_units, const uint32_t *umax,
+                                                           uint32_t stride, FConsts K) {
+  // The host launches enough levels for the worst case (one bucket holding a
+  // window's every point); a level is needed only if some bucket has more
+  // than `stride` units.
+  if (stride >= *umax) return;
   const uint32_t total = *total_units;
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
     const uint32_t b = ubucket[u];
