@@ -47,6 +47,8 @@ SIGNATURES = {
     "zp_msm_mult_device": (_i32, [_vp, _vp, _u64, _vp, _vp]),
     "zp_msm_mult_prefix_device": (_i32, [_vp, _sz, _vp, _vp, _vp]),
     "zp_msm_num_points": (_sz, [_vp]),
+    "zp_msm_precompute_width": (_i32, [_vp]),
+    "zp_debug_msm_plan": (_i32, [_vp, _sz, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "zp_msm_num_nonconvertible": (_sz, [_vp]),
     "zp_msm_workspace_create": (_i32, [ctypes.POINTER(_vp)]),
     "zp_msm_workspace_destroy": (_i32, [_vp]),

@@ -149,4 +149,76 @@ ZP_HD Xyzt ext_from_pre(const PrePt &p, bool neg, const FConsts &K) {
   return r;
 }
 
+// ---- Weierstrass (y^2 = x^3 + 1) Jacobian arithmetic: the points without a
+// twisted-Edwards image and the final result (bls12_377_g1.h:493-566) ----
+struct Jac {
+  F X, Y, Z;  // Z == 0: infinity
+};
+
+ZP_HD bool f_zero_c(const F &a) { return f_is_zero_canon(a); }
+
+ZP_HD void jac_dbl(Jac &p, const FConsts &K) {
+  if (f_zero_c(p.Z)) return;
+  F A = f_sqr(p.X), B = f_sqr(p.Y), C = f_sqr(B);
+  F t = f_sqr(f_add(p.X, B));
+  t = f_sub_kq(f_sub_kq(t, A, K.q2), C, K.q2);
+  F D = f_canonical(f_add(t, t));
+  F E = f_canonical(f_add(f_add(A, A), A));
+  F Fv = f_sqr(E);
+  F X3 = f_canonical(f_sub_kq(f_sub_kq(Fv, D, K.q2), D, K.q2));
+  F C8 = f_canonical(f_add(C, C));
+  C8 = f_canonical(f_add(C8, C8));
+  C8 = f_canonical(f_add(C8, C8));
+  F Y3 = f_sub_kq(f_mul(E, f_sub_kq(D, X3, K.q2)), C8, K.q2);
+  F Z3 = f_mul(p.Y, p.Z);
+  Z3 = f_add(Z3, Z3);
+  p.X = f_canonical(X3);
+  p.Y = f_canonical(Y3);
+  p.Z = f_canonical(Z3);
+}
+
+// p += (x2, y2) affine.
+ZP_HD void jac_add_aff(Jac &p, const F &x2, const F &y2, const FConsts &K) {
+  if (f_zero_c(p.Z)) {
+    p.X = x2;
+    p.Y = y2;
+    p.Z = K.one;
+    return;
+  }
+  F Z1Z1 = f_sqr(p.Z);
+  F U2 = f_mul(x2, Z1Z1);
+  F S2 = f_mul(f_mul(y2, p.Z), Z1Z1);
+  F H = f_canonical(f_sub_kq(U2, p.X, K.q2));
+  F r = f_canonical(f_sub_kq(S2, p.Y, K.q2));
+  if (f_zero_c(H)) {
+    if (f_zero_c(r))
+      jac_dbl(p, K);
+    else
+      p.Z = f_zero();
+    return;
+  }
+  F HH = f_sqr(H), HHH = f_mul(HH, H), V = f_canonical(f_mul(p.X, HH));
+  F X3 = f_sqr(r);
+  X3 = f_canonical(f_sub_kq(X3, HHH, K.q2));
+  X3 = f_canonical(f_sub_kq(X3, V, K.q2));
+  X3 = f_canonical(f_sub_kq(X3, V, K.q2));
+  F Y3 = f_sub_kq(f_mul(r, f_sub_kq(V, X3, K.q2)), f_mul(p.Y, HHH), K.q2);
+  p.X = X3;
+  p.Y = f_canonical(Y3);
+  p.Z = f_canonical(f_mul(p.Z, H));
+}
+
+ZP_HD void jac_add_jac(Jac &p, const Jac &q, const FConsts &K) {
+  if (f_zero_c(q.Z)) return;
+  if (f_zero_c(p.Z)) {
+    p = q;
+    return;
+  }
+  // Normalise q to affine (rare path: a handful of points per call).
+  F zi = f_inv(q.Z, K.one), zi2 = f_sqr(zi);
+  F x = f_canonical(f_mul(q.X, zi2));
+  F y = f_canonical(f_mul(q.Y, f_mul(zi2, zi)));
+  jac_add_aff(p, x, y, K);
+}
+
 }  // namespace zp

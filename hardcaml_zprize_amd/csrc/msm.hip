@@ -686,6 +686,20 @@ static double msm_cost(uint64_t n, const HostLayout &L, bool shared) {
   return (double)n * L.W + 2.6 * (shared ? (double)maxnb : buckets);
 }
 
+// Whether a job of n points over table cx takes the window-shifted table
+// (ONE window of n * W virtual points, every window adding into one shared
+// bucket set of 2^(c-1) buckets): only when the table has the copies, no
+// window is forced, and it costs fewer bucket additions than the per-window
+// layout `per_window` (small prefixes of a big table do not: the shared
+// bucket set outweighs their points).  LS receives the shifted layout.
+static bool job_uses_shifted(const zp_msm_context *cx, size_t n, bool window_forced, const HostLayout &per_window,
+                             HostLayout &LS) {
+  if (!cx->d_shift || window_forced || !n) return false;
+  LS = layout_uniform(cx->shift_c);
+  layout_finish(LS);
+  return msm_cost(n, LS, true) < msm_cost(n, per_window, false);
+}
+
 // Host description of one job of a batch.
 struct HostJob {
   zp_msm_context *ctx;
@@ -725,13 +739,8 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
     // table entry 2^(c w) P_i) when that costs fewer bucket additions than the
     // per-window layout (small prefixes of a big table do not: one shared
     // bucket set of 2^(c-1) buckets outweighs their points).
-    bool pre = false;
     HostLayout LS;
-    if (cx->d_shift && !h.window && !bopts.window && !bopts.reference && h.n) {
-      LS = layout_uniform(cx->shift_c);
-      layout_finish(LS);
-      pre = msm_cost(h.n, LS, true) < msm_cost(h.n, layouts[j], false);
-    }
+    const bool pre = job_uses_shifted(cx, h.n, h.window || bopts.window || bopts.reference, layouts[j], LS);
     MsmJobDev &d = jd[j];
     d.conv = cx->d_conv;
     d.scalars = h.d_scalars;
@@ -1529,6 +1538,20 @@ extern "C" int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs,
 }
 
 extern "C" size_t zp_msm_num_points(const zp_msm_context *ctx) { return ctx ? ctx->n : 0; }
+extern "C" int zp_msm_precompute_width(const zp_msm_context *ctx) { return ctx && ctx->d_shift ? ctx->shift_c : 0; }
+
+extern "C" int zp_debug_msm_plan(const zp_msm_context *ctx, size_t n, int *shifted, int *window_bits,
+                                 int *num_windows) {
+  ZP_CHECK(ctx && shifted && window_bits && num_windows, ZP_ERR_INVALID_ARG, "zp_debug_msm_plan: null argument");
+  ZP_CHECK(n <= ctx->n, ZP_ERR_INVALID_ARG, "zp_debug_msm_plan: n=%zu > %zu points", n, ctx->n);
+  const HostLayout L = choose_layout(ctx->opts, n);
+  HostLayout LS;
+  const bool pre = job_uses_shifted(ctx, n, false, L, LS);
+  *shifted = pre;
+  *window_bits = pre ? ctx->shift_c : (ctx->opts.reference ? 13 : L.width[0]);
+  *num_windows = pre ? LS.W : L.W;
+  return ZP_OK;
+}
 extern "C" size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx) { return ctx ? ctx->num_nonconv : 0; }
 
 // ---------------------------------------------------------- generator --

@@ -58,6 +58,17 @@ class MsmContext:
     def __exit__(self, *exc):
         self.close()
 
+    def precompute_width(self):
+        """Width of the window-shifted table (init option precompute=), 0 if none."""
+        return int(lib().zp_msm_precompute_width(self.ctx))
+
+    def plan(self, n=None):
+        """(shifted, window_bits, num_windows) of an MSM over the first n points."""
+        sh, wb, nw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        call("zp_debug_msm_plan", self.ctx, self.n if n is None else int(n), ctypes.byref(sh), ctypes.byref(wb),
+             ctypes.byref(nw))
+        return bool(sh.value), wb.value, nw.value
+
     def num_nonconvertible(self):
         return int(lib().zp_msm_num_nonconvertible(self.ctx))
 
@@ -204,24 +215,3 @@ def debug_digits(scalars, options=""):
     call("zp_debug_msm_digits", opt, len(opt), sc.ctypes.data_as(ctypes.c_void_p), len(sc),
          out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(W))
     return out
-
-
-def smoke_check():
-    """One small MSM on the GPU, checked against the CPU oracle (smoke only)."""
-    import oracle  # the checker
-
-    rng = np.random.default_rng(5)
-    n = 64
-    ks = rng.integers(0, 2**63, size=(n, 4), dtype=np.uint64)
-    ks[:, 3] &= np.uint64((1 << 60) - 1)
-    pts = generate_points(ks)
-    exp_pts = oracle.mul_generator_many(ks)
-    for a, b in zip(pts, exp_pts):
-        assert oracle.affine_equal(a, b), "generated point mismatch"
-    sc = rng.integers(0, 2**63, size=(2 * n, 4), dtype=np.uint64)
-    sc[:, 3] &= np.uint64((1 << 60) - 1)
-    with MsmContext(pts) as ctx:
-        got = ctx.mult(sc, 2)
-    exp = oracle.msm(pts, sc, 2)
-    assert got.tobytes() == exp.tobytes(), "MSM mismatch vs oracle"
-    print("smoke: MSM 64 points x 2 batches bit-exact vs oracle", flush=True)

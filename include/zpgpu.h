@@ -183,6 +183,8 @@ int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_b
 int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_projective_t *out, const biginteger256_t *d_scalars,
                               void *stream);
 size_t zp_msm_num_points(const zp_msm_context *ctx);
+/* Width c of the context's window-shifted table ("precompute="), 0 if it has none. */
+int zp_msm_precompute_width(const zp_msm_context *ctx);
 
 /* Batched form: many MSMs of mixed sizes, each over a prefix of a resident
  * table (tables may differ between jobs), run as ONE launch sequence: the
@@ -224,6 +226,11 @@ int zp_debug_fq_mul(const uint64_t *a, const uint64_t *b, uint64_t *out, size_t 
  * (digits may be NULL to query W). */
 int zp_debug_msm_digits(const char *options, size_t options_len, const biginteger256_t *scalars, size_t n,
                         int32_t *digits, int *num_windows);
+/* The plan an MSM over the first n points of ctx's table takes: *shifted = 1
+ * when it runs as one window over the window-shifted table (shared buckets),
+ * *window_bits = its window width (the first window's for the reference
+ * layout), *num_windows = the digit windows per scalar. */
+int zp_debug_msm_plan(const zp_msm_context *ctx, size_t n, int *shifted, int *window_bits, int *num_windows);
 /* The resident precomputed points [i0, i0 + count) as wire-form triples
  * (y - x, y + x, 2d x y) of the twisted-Edwards image, 18 words each. */
 int zp_debug_msm_precomputed(zp_msm_context *ctx, size_t i0, size_t count, uint64_t *out);

@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 from hardcaml_zprize_amd import msm as zmsm
+from hardcaml_zprize_amd import workload
 
 pytestmark = pytest.mark.gpu
 Q, R = oracle.Q, oracle.R_ORDER
@@ -212,10 +213,8 @@ def test_large_closed_form(log_n, options):
     _say(f"msm large 2^{log_n} {options}")
     rng = np.random.default_rng(2024 + log_n)
     n = 1 << log_n
-    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    ks[:, 3] &= np.uint64((1 << 60) - 1)  # < 2^252 < r
-    sc = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    sc[:, 3] &= np.uint64((1 << 60) - 1)
+    ks = workload.rand_scalars(rng, n, nonzero=True)  # uniform in [1, r)
+    sc = workload.rand_scalars(rng, n)  # uniform in [0, r), bit 252 included
     dk = zmsm.DeviceBuffer(32 * n).upload(ks)
     dp = zmsm.DeviceBuffer(104 * n)
     zmsm.generate_points_device(dk.ptr, dp.ptr, n)
@@ -412,12 +411,10 @@ def test_host_scalars_pinned_upload_batches():
     _say("msm pinned upload 3 x 2^21")
     rng = np.random.default_rng(77)
     n = 1 << 21
-    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    ks[:, 3] &= np.uint64((1 << 60) - 1)
+    ks = workload.rand_scalars(rng, n, nonzero=True)
     raw = np.zeros(3 * n * 32 + 8, dtype=np.uint8)
     sc = raw[8:].view(np.uint64).reshape(3 * n, 4)  # 8-byte offset from the allocation
-    sc[:] = rng.integers(0, 2**64, size=(3 * n, 4), dtype=np.uint64)
-    sc[:, 3] &= np.uint64((1 << 60) - 1)
+    sc[:] = workload.rand_scalars(rng, 3 * n)
     dk = zmsm.DeviceBuffer(32 * n).upload(ks)
     dp = zmsm.DeviceBuffer(104 * n)
     zmsm.generate_points_device(dk.ptr, dp.ptr, n)

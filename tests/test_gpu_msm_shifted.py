@@ -13,6 +13,7 @@ import pytest
 
 import oracle
 from hardcaml_zprize_amd import msm as zmsm
+from hardcaml_zprize_amd import workload
 
 pytestmark = pytest.mark.gpu
 R = oracle.R_ORDER
@@ -117,21 +118,24 @@ def test_shifted_table_prefixes_and_batch():
         d.free()
 
 
-def test_shifted_table_large_closed_form():
-    """2^20 points generated on the GPU, automatic width (18 bits, 15 shifted
-    tables); result vs (sum s_i k_i mod r) G."""
-    _say("msm shifted 2^20 closed form")
-    rng = np.random.default_rng(904)
-    n = 1 << 20
-    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    ks[:, 3] &= np.uint64((1 << 60) - 1)
-    sc = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    sc[:, 3] &= np.uint64((1 << 60) - 1)
+@pytest.mark.parametrize("log_n,options,width", [(20, "precompute=auto", 17), (22, "precompute=23", 23)])
+def test_shifted_table_large_closed_form(log_n, options, width):
+    """Points generated on the GPU, scalars uniform in [0, r) (bit 252
+    included); result vs (sum s_i k_i mod r) G.  2^20 at the automatic width
+    (17 bits, 15 shifted copies, 2^16 shared buckets); 2^22 over a 23-bit
+    table (11 copies, 2^22 shared buckets in 2^11 sort groups), which the
+    cost model takes over the per-window layout at this size."""
+    _say(f"msm shifted 2^{log_n} {options} closed form")
+    rng = np.random.default_rng(904 + log_n)
+    n = 1 << log_n
+    ks = workload.rand_scalars(rng, n, nonzero=True)
+    sc = workload.rand_scalars(rng, n)
     dk = zmsm.DeviceBuffer(32 * n).upload(ks)
     dp = zmsm.DeviceBuffer(104 * n)
     zmsm.generate_points_device(dk.ptr, dp.ptr, n)
     try:
-        with zmsm.MsmContext((dp.ptr, n), "precompute=auto") as ctx:
+        with zmsm.MsmContext((dp.ptr, n), options) as ctx:
+            assert ctx.plan() == (True, width, (253 + width - 1) // width)
             out = ctx.mult(sc, 1)
         e = oracle.dot_mod_r(sc, ks)
         assert oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(oracle.generator(), e))

@@ -38,3 +38,10 @@ def test_fr_to_montgomery_host(tmp_path):
     """fr253.hpp (the harness-file scalar conversion s -> s 2^256 mod r) against
     shift-and-subtract big-integer arithmetic."""
     _build_and_run(tmp_path, "fr_check")
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+def test_fq_binary_inverse_host(tmp_path):
+    """fq377.hpp f_inv_fast (the device finalisation's inversion) against
+    Fermat inversion and a * a^-1 = 1."""
+    _build_and_run(tmp_path, "fq_inv_check")

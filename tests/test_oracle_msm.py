@@ -127,3 +127,62 @@ def test_planted_mistake_is_caught():
     out = oracle.msm(pts, _scalar_words(sc), 1)
     wrong = sum(s * k for s, k in zip(sc[1:], ks[1:])) % R
     assert not oracle.affine_equal(oracle.projective_to_affine(out[0]), oracle.g1_scalar_mul(oracle.generator(), wrong))
+
+
+def _digits(s):
+    mag, neg = oracle.msm_digits(s)
+    return [-int(m) if n else int(m) for m, n in zip(mag, neg)]
+
+
+def test_digit_tie_rule():
+    """A window slice (plus carry) of exactly 2^(w-1) becomes -2^(w-1) with a
+    carry into the next window, and one of exactly 2^w becomes 0 with a carry
+    (zprize/msm_pippenger/hardcaml/test/top/utils.ml:261-272: `>=`, not `>`).
+    Expected digits written out by hand from that rule."""
+    z = [0] * 20
+    assert _digits(1 << 12) == [-4096, 1] + z[2:]            # slice 4096 = 2^12: the tie
+    assert _digits((1 << 12) - 1) == [4095] + z[1:]          # one below the tie stays positive
+    assert _digits((1 << 13) - 1) == [-1, 1] + z[2:]
+    # window 1 reaches the tie only through the carry: 4095 + 1
+    assert _digits((((1 << 12) - 1) << 13) | (1 << 12)) == [-4096, -4096, 1] + z[3:]
+    # window 1 overflows to 2^13 through the carry: digit 0, carry on
+    assert _digits((((1 << 13) - 1) << 13) | ((1 << 13) - 1)) == [-1, 0, 1] + z[3:]
+    # the 12-bit windows (13..18) tie at 2^11
+    assert _digits(1 << (13 * 13 + 11)) == z[:13] + [-2048, 1] + z[15:]
+    # the top window (offset 241) is unsigned: slice + carry as is
+    top = 2389 << 241  # (r - 1) >> 241
+    assert _digits(top) == z[:19] + [2389]
+
+
+def test_dot_mod_r_vs_python_ints():
+    """orc_dot_mod_r (the closed form behind every large MSM check) against
+    Python integers, on random and extreme operands (all-ones words: the
+    640-bit accumulator's top words)."""
+    rng = random.Random(31)
+    for n in (1, 2, 7, 1000):
+        s = [rng.randrange(2**256) for _ in range(n)]
+        k = [rng.randrange(2**256) for _ in range(n)]
+        got = oracle.dot_mod_r(_scalar_words(s), _scalar_words(k))
+        assert got == sum(a * b for a, b in zip(s, k)) % R
+    ones = [2**256 - 1] * 4096
+    assert oracle.dot_mod_r(_scalar_words(ones), _scalar_words(ones)) == 4096 * (2**256 - 1) ** 2 % R
+    # a planted slip (one product dropped) changes the value
+    s = [rng.randrange(R) for _ in range(5)]
+    k = [rng.randrange(R) for _ in range(5)]
+    assert oracle.dot_mod_r(_scalar_words(s), _scalar_words(k)) != sum(a * b for a, b in zip(s[1:], k[1:])) % R
+
+
+def test_numpy_dot_mod_r_matches_oracle():
+    """The benchmark workload's own exact dot product (float64 limb products)
+    equals the oracle's and Python's."""
+    import numpy as np
+
+    from hardcaml_zprize_amd import workload
+
+    rng = np.random.default_rng(32)
+    for n in (1, 3, (1 << 20) + 17):
+        s = workload.rand_scalars(rng, n)
+        k = workload.rand_scalars(rng, n, nonzero=True)
+        assert workload.dot_mod_r(s, k) == oracle.dot_mod_r(s, k)
+    s = np.full((9, 4), 2**64 - 1, dtype=np.uint64)
+    assert workload.dot_mod_r(s, s) == 9 * (2**256 - 1) ** 2 % R
