@@ -286,19 +286,59 @@ ZP_HD F f_inv(const F &a, const F &one) {
   return r;
 }
 
-// ---- inversion by the binary extended Euclidean algorithm ----
-// On plain 384-bit integers in 12 little-endian 32-bit words.
-ZP_HD bool w12_is_one(const uint32_t (&a)[12]) {
-  uint32_t acc = a[0] ^ 1u;
+// ---- inversion: binary GCD with batched divsteps ----
+// T. Pornin, "Optimized Binary GCD for Modular Inversion" (IACR ePrint
+// 2020/972), Algorithm 2 with k = 32: each outer step runs 31 binary-GCD
+// steps on 64-bit approximations of a and b (low 31 bits and top 33 bits),
+// collecting the update factors f0, g0, f1, g1 (|.| <= 2^31), then applies
+// them to the full values: a' = (a f0 + b g0) / 2^31, b' = (a f1 + b g1) / 2^31
+// (exact), and likewise u', v' with the division by 2^31 done mod q by adding
+// k q (q = 1 mod 2^46, so k = -t mod 2^31).  Invariants a = u y, b = v y
+// (mod q); 25 outer steps (775 >= 2 * 377 - 1 divsteps) end with b = 1, so
+// v = y^-1.  Values are 13 limbs of 30 bits (signed 64-bit accumulators).
+static constexpr int INV_L = 13;
+static constexpr uint32_t INV_M30 = (1u << 30) - 1;
+
+// r = (x f + y g) as a signed 13-limb value plus a signed top carry.
+ZP_HD int64_t inv_lin(const uint32_t (&x)[INV_L], const uint32_t (&y)[INV_L], int64_t f, int64_t g,
+                      uint32_t (&r)[INV_L]) {
+  int64_t c = 0;
 #pragma unroll
-  for (int i = 1; i < 12; i++) acc |= a[i];
-  return acc == 0;
+  for (int i = 0; i < INV_L; i++) {
+    c += (int64_t)x[i] * f + (int64_t)y[i] * g;
+    r[i] = (uint32_t)c & INV_M30;
+    c >>= 30;  // arithmetic
+  }
+  return c;
 }
 
-ZP_HD bool w12_geq(const uint32_t (&a)[12], const uint32_t (&b)[12]) {
+// r = -r (value r + top 2^390) when top < 0; returns whether it negated.
+ZP_HD bool inv_abs(uint32_t (&r)[INV_L], int64_t &top) {
+  if (top >= 0) return false;
+  int64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < INV_L; i++) {
+    c -= (int64_t)r[i];
+    r[i] = (uint32_t)c & INV_M30;
+    c >>= 30;
+  }
+  top = -top + c;
+  return true;
+}
+
+// r = (r + top 2^390) >> 31 for a non-negative value (r[0] bits 0..29 and
+// r[1] bit 0 are zero).
+ZP_HD void inv_shr31(uint32_t (&r)[INV_L], int64_t top) {
+#pragma unroll
+  for (int i = 0; i < INV_L - 2; i++) r[i] = (r[i + 1] >> 1) | ((r[i + 2] & 1u) << 29);
+  r[INV_L - 2] = (r[INV_L - 1] >> 1) | (((uint32_t)top & 1u) << 29);
+  r[INV_L - 1] = (uint32_t)(top >> 1);
+}
+
+ZP_HD bool inv_geq(const uint32_t (&a)[INV_L], const uint32_t (&b)[INV_L]) {
   bool decided = false, geq = true;
 #pragma unroll
-  for (int i = 11; i >= 0; i--) {
+  for (int i = INV_L - 1; i >= 0; i--) {
     if (!decided && a[i] != b[i]) {
       decided = true;
       geq = a[i] > b[i];
@@ -307,91 +347,168 @@ ZP_HD bool w12_geq(const uint32_t (&a)[12], const uint32_t (&b)[12]) {
   return geq;
 }
 
-// a -= b (a >= b).
-ZP_HD void w12_sub(uint32_t (&a)[12], const uint32_t (&b)[12]) {
-  uint64_t br = 0;
+ZP_HD void inv_addsub(uint32_t (&a)[INV_L], const uint32_t (&b)[INV_L], bool sub) {
+  int64_t c = 0;
 #pragma unroll
-  for (int i = 0; i < 12; i++) {
-    const uint64_t d = (uint64_t)a[i] - b[i] - br;
-    a[i] = (uint32_t)d;
-    br = (d >> 32) & 1;
+  for (int i = 0; i < INV_L; i++) {
+    c += sub ? (int64_t)a[i] - (int64_t)b[i] : (int64_t)a[i] + (int64_t)b[i];
+    a[i] = (uint32_t)c & INV_M30;
+    c >>= 30;
   }
 }
 
-// a = (a + c) / 2 with the carry out of the top word kept (c = 0 or q).
-ZP_HD void w12_add_half(uint32_t (&a)[12], const uint32_t (&c)[12]) {
-  uint64_t cy = 0;
+// (x f + y g) / 2^31 mod q, for x, y in [0, q).
+ZP_HD void inv_lin_mod(const uint32_t (&x)[INV_L], const uint32_t (&y)[INV_L], int64_t f, int64_t g,
+                       const uint32_t (&q)[INV_L], uint32_t (&r)[INV_L]) {
+  int64_t top = inv_lin(x, y, f, g, r);
+  // + k q with k = -(r mod 2^31) mod 2^31 makes the low 31 bits zero
+  const int64_t k = (int64_t)((0u - (r[0] | ((r[1] & 1u) << 30))) & 0x7fffffffu);
+  int64_t c = 0;
 #pragma unroll
-  for (int i = 0; i < 12; i++) {
-    const uint64_t s = (uint64_t)a[i] + c[i] + cy;
-    a[i] = (uint32_t)s;
-    cy = s >> 32;
+  for (int i = 0; i < INV_L; i++) {
+    c += (int64_t)r[i] + (int64_t)q[i] * k;
+    r[i] = (uint32_t)c & INV_M30;
+    c >>= 30;
   }
+  top += c;
+  // the value (|.| < 3 q after the shift) as sign and magnitude
+  const bool neg = inv_abs(r, top);
+  inv_shr31(r, top);
+  if (neg) {  // r = q - |r| mod q, |r| < 3q
+    for (int t = 0; t < 3 && inv_geq(r, q); t++) inv_addsub(r, q, true);
+    bool z = true;
 #pragma unroll
-  for (int i = 0; i < 11; i++) a[i] = (a[i] >> 1) | (a[i + 1] << 31);
-  a[11] = (a[11] >> 1) | ((uint32_t)cy << 31);
+    for (int i = 0; i < INV_L; i++) z = z && r[i] == 0;
+    if (!z) {
+      uint32_t m[INV_L];
+#pragma unroll
+      for (int i = 0; i < INV_L; i++) m[i] = q[i];
+      inv_addsub(m, r, true);
+#pragma unroll
+      for (int i = 0; i < INV_L; i++) r[i] = m[i];
+    }
+  } else {
+    for (int t = 0; t < 3 && inv_geq(r, q); t++) inv_addsub(r, q, true);
+  }
 }
 
-// a = a - b mod q for a, b < q.
-ZP_HD void w12_sub_mod(uint32_t (&a)[12], const uint32_t (&b)[12], const uint32_t (&q)[12]) {
-  if (!w12_geq(a, b)) {
-    uint64_t cy = 0;  // a + q < 2^378: no carry out of the top word
+ZP_HD int inv_bitlen(const uint32_t (&a)[INV_L]) {
+  int len = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) {
-      const uint64_t s = (uint64_t)a[i] + q[i] + cy;
-      a[i] = (uint32_t)s;
-      cy = s >> 32;
+  for (int i = 0; i < INV_L; i++)
+    if (a[i]) len = 30 * i + 32 - __builtin_clz(a[i]);
+  return len;
+}
+
+// Bits [pos, pos + 33) of a (pos >= 0), as an integer < 2^33.
+ZP_HD uint64_t inv_bits33(const uint32_t (&a)[INV_L], int pos) {
+  const int l = pos / 30, s = pos % 30;
+  uint64_t w = 0;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int i = l + j;
+    const uint64_t v = (i < INV_L) ? a[i] : 0;
+    w |= v << (30 * j);
+  }
+  return (w >> s) & ((1ull << 33) - 1);
+}
+
+// 1/a in the R' domain for a != 0 (mod q): with y = x R' the canonical
+// integer, y^-1 R'^3 / R' = x^-1 R'.  Sets out = 0 for a = 0 mod q.  Returns
+// false if the 25 outer steps did not reach b = 1 (not expected for any a;
+// the caller then falls back to f_inv).
+ZP_HD bool f_inv_pornin(const F &a, const FConsts &K, F &out) {
+  uint32_t q[INV_L], A[INV_L], B[INV_L], U[INV_L], V[INV_L];
+  {
+    F qf;
+#pragma unroll
+    for (int i = 0; i < 14; i++) qf.l[i] = q28(i);
+    const F ac = f_canonical(a);
+    // 28-bit limbs -> 30-bit limbs
+#pragma unroll
+    for (int i = 0; i < INV_L; i++) {
+      const int bit = 30 * i, l = bit / 28, s = bit % 28;
+      const uint64_t w0 = (uint64_t)ac.l[l] | ((l + 1 < 14) ? (uint64_t)ac.l[l + 1] << 28 : 0) |
+                          ((l + 2 < 14) ? (uint64_t)ac.l[l + 2] << 56 : 0);
+      const uint64_t w1 = (uint64_t)qf.l[l] | ((l + 1 < 14) ? (uint64_t)qf.l[l + 1] << 28 : 0) |
+                          ((l + 2 < 14) ? (uint64_t)qf.l[l + 2] << 56 : 0);
+      A[i] = (uint32_t)(w0 >> s) & INV_M30;
+      q[i] = (uint32_t)(w1 >> s) & INV_M30;
+      B[i] = q[i];
+      U[i] = i == 0;
+      V[i] = 0;
     }
   }
-  w12_sub(a, b);
+  bool nz = false;
+#pragma unroll
+  for (int i = 0; i < INV_L; i++) nz = nz || A[i] != 0;
+  if (!nz) {
+    out = f_zero();
+    return true;
+  }
+  for (int it = 0; it < 25; it++) {
+    const int la = inv_bitlen(A), lb = inv_bitlen(B);
+    const int n = la > lb ? (la > 64 ? la : 64) : (lb > 64 ? lb : 64);
+    uint64_t a_ = (uint64_t)(A[0] | ((A[1] & 1u) << 30)) | (inv_bits33(A, n - 33) << 31);
+    uint64_t b_ = (uint64_t)(B[0] | ((B[1] & 1u) << 30)) | (inv_bits33(B, n - 33) << 31);
+    int64_t f0 = 1, g0 = 0, f1 = 0, g1 = 1;
+    for (int j = 0; j < 31; j++) {
+      const uint64_t odd = 0ull - (a_ & 1);
+      const uint64_t sw = odd & (0ull - (uint64_t)(a_ < b_));
+      // conditional swap (a_, b_), (f0, f1), (g0, g1)
+      const uint64_t tab = (a_ ^ b_) & sw;
+      a_ ^= tab;
+      b_ ^= tab;
+      const int64_t tf = (f0 ^ f1) & (int64_t)sw, tg = (g0 ^ g1) & (int64_t)sw;
+      f0 ^= tf;
+      f1 ^= tf;
+      g0 ^= tg;
+      g1 ^= tg;
+      // conditional subtract
+      a_ -= b_ & odd;
+      f0 -= f1 & (int64_t)odd;
+      g0 -= g1 & (int64_t)odd;
+      a_ >>= 1;
+      f1 <<= 1;
+      g1 <<= 1;
+    }
+    uint32_t nA[INV_L], nB[INV_L];
+    int64_t ta = inv_lin(A, B, f0, g0, nA), tb = inv_lin(A, B, f1, g1, nB);
+    if (inv_abs(nA, ta)) f0 = -f0, g0 = -g0;
+    if (inv_abs(nB, tb)) f1 = -f1, g1 = -g1;
+    inv_shr31(nA, ta);
+    inv_shr31(nB, tb);
+    uint32_t nU[INV_L], nV[INV_L];
+    inv_lin_mod(U, V, f0, g0, q, nU);
+    inv_lin_mod(U, V, f1, g1, q, nV);
+#pragma unroll
+    for (int i = 0; i < INV_L; i++) {
+      A[i] = nA[i];
+      B[i] = nB[i];
+      U[i] = nU[i];
+      V[i] = nV[i];
+    }
+  }
+  bool one = B[0] == 1;
+#pragma unroll
+  for (int i = 1; i < INV_L; i++) one = one && B[i] == 0;
+  if (!one) return false;
+  // 30-bit limbs -> 28-bit limbs of v = y^-1, then * R'^3 / R'
+  F v;
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    const int bit = 28 * i, l = bit / 30, s = bit % 30;
+    const uint64_t w = (uint64_t)V[l] | ((l + 1 < INV_L) ? (uint64_t)V[l + 1] << 30 : 0);
+    v.l[i] = (uint32_t)(w >> s) & F_MASK;
+  }
+  out = f_mul(v, K.r3);
+  return true;
 }
 
-// 1/a in the R' domain for a != 0 (mod q): the binary extended Euclidean
-// algorithm on the canonical integer v = x R' gives v^-1 mod q, and one
-// multiply by R'^3 turns it into x^-1 R'.  About 1,100 shift/subtract steps
-// on 12-word integers instead of the ~570 multiplications of f_inv (a^(q-2));
-// not constant time (the values are public).  Returns 0 for a = 0 mod q.
 ZP_HD F f_inv_fast(const F &a, const FConsts &K) {
-  F qf;
-#pragma unroll
-  for (int i = 0; i < 14; i++) qf.l[i] = q28(i);
-  const Packed qp = f_pack(qf);
-  const Packed ap = f_pack(f_canonical(a));
-  uint32_t q[12], u[12], v[12], x1[12], x2[12], zero[12];
-#pragma unroll
-  for (int i = 0; i < 12; i++) {
-    q[i] = qp.w[i];
-    u[i] = ap.w[i];
-    v[i] = qp.w[i];
-    x1[i] = i == 0;
-    x2[i] = 0;
-    zero[i] = 0;
-  }
-  uint32_t nz = 0;
-#pragma unroll
-  for (int i = 0; i < 12; i++) nz |= u[i];
-  if (!nz) return f_zero();
-  // Invariants: x1 a = u, x2 a = v (mod q); gcd(u, v) = 1 since q is prime.
-  // Every step halves u or v or replaces the larger by the difference, so
-  // u + v < 2^379 falls to 2 within 2 * 379 halvings; the bounds below are
-  // never reached for a != 0.
-  for (int it = 0; it < 1600 && !w12_is_one(u) && !w12_is_one(v); it++) {
-    for (int k = 0; k < 384 && !(u[0] & 1); k++) w12_add_half(u, zero), w12_add_half(x1, (x1[0] & 1) ? q : zero);
-    for (int k = 0; k < 384 && !(v[0] & 1); k++) w12_add_half(v, zero), w12_add_half(x2, (x2[0] & 1) ? q : zero);
-    if (w12_is_one(u) || w12_is_one(v)) break;
-    if (w12_geq(u, v)) {
-      w12_sub(u, v);
-      w12_sub_mod(x1, x2, q);
-    } else {
-      w12_sub(v, u);
-      w12_sub_mod(x2, x1, q);
-    }
-  }
-  Packed rp;
-  const bool u1 = w12_is_one(u);
-#pragma unroll
-  for (int i = 0; i < 12; i++) rp.w[i] = u1 ? x1[i] : x2[i];
-  return f_mul(f_unpack(rp), K.r3);
+  F r;
+  if (!f_inv_pornin(a, K, r)) r = f_inv(a, K.one);
+  return r;
 }
 
 }  // namespace zp

@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "msm_host.hpp"
 #include "msm_kernels.hpp"
+#include "msm_finalize.hpp"
 #include "fr253.hpp"
 
 namespace zp {
@@ -103,6 +104,12 @@ struct MsmOptions {
   // uniform layout of c bits ("precompute=<c>" or "precompute=auto"); 0 = off.
   int precompute = 0;
   bool bad_precompute = false;
+  // Batch option "finalize=host": the window combine, the exceptional points
+  // (by double-and-add over the exact scalar sums) and the wire conversion
+  // run on the host instead of k_msm_finalize (a cross-check of the device
+  // path; the results are the same bytes).
+  bool host_finalize = false;
+  bool bad_finalize = false;
 };
 
 static MsmOptions parse_options(const char *s, size_t len) {
@@ -133,6 +140,9 @@ static MsmOptions parse_options(const char *s, size_t len) {
         o.bad_sort = o.sort < 0;
       } else if (k == "sort_lbits") {
         o.sort_lbits = atoi(v.c_str());
+      } else if (k == "finalize") {
+        o.host_finalize = v == "host";
+        o.bad_finalize = v != "host" && v != "device";
       } else if (k == "precompute") {
         o.precompute = v == "auto" ? -1 : v == "off" ? 0 : atoi(v.c_str());
         o.bad_precompute = v != "auto" && v != "off" && (o.precompute < 8 || o.precompute > 24);
@@ -151,6 +161,7 @@ static int validate_options(const MsmOptions &o, const char *who) {
   ZP_CHECK(!o.bad_sort, ZP_ERR_INVALID_ARG, "%s: sort= must be auto, lds, msd or atomic", who);
   ZP_CHECK(o.sort_lbits <= 12, ZP_ERR_INVALID_ARG, "%s: sort_lbits=%d > 12", who, o.sort_lbits);
   ZP_CHECK(!o.bad_precompute, ZP_ERR_INVALID_ARG, "%s: precompute= must be auto, off or a width in [8, 24]", who);
+  ZP_CHECK(!o.bad_finalize, ZP_ERR_INVALID_ARG, "%s: finalize= must be device or host", who);
   return ZP_OK;
 }
 
@@ -430,11 +441,18 @@ struct zp_msm_workspace {
   uint64_t *nc_sums = nullptr;  // 5 words per exceptional-point group per job
   size_t cap_nc = 0;
   std::vector<uint64_t> h_nc_sums;
+  FinJobDev *fin = nullptr;  // per-job finalisation descriptors
+  size_t cap_fin = 0;
+  uint64_t *d_out = nullptr;  // per-job results, 18 words (g1_projective_t) each
+  size_t cap_out = 0;
+  std::vector<uint64_t> h_out;
+  FConsts *d_consts = nullptr;  // the field constants in device memory (k_msm_finalize)
+  size_t cap_consts = 0;
 
   void release() {
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
                   groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
-                  winpart, umax};
+                  winpart, umax, fin, d_out, d_consts};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = zp_msm_workspace();
@@ -545,6 +563,7 @@ struct zp_msm_context {
   size_t num_nonconv = 0;
   uint32_t *d_nc_idx = nullptr;  // every group's indices, group after group
   uint32_t *d_nc_off = nullptr;  // nc_groups.size() + 1 offsets into d_nc_idx
+  NcMultDev *d_nc_mult = nullptr;  // P, 2P, 3P of each group (k_msm_finalize)
   // Window-shifted table (init option "precompute"): d_shift[i * shift_W + w]
   // = 2^(shift_c w) P_i; null when not requested.
   PackedPre *d_shift = nullptr;
@@ -1061,8 +1080,6 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
                        ws.gstart, ws.winsum, KC);
   }
   ZP_HIP(hipGetLastError());
-  ws.h_winsum.resize(Wt);
-  ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
   // exceptional-point scalar sums, per job and group
   std::vector<size_t> nc0(count + 1, 0);
   for (size_t j = 0; j < count; j++) nc0[j + 1] = nc0[j] + jobs[j].ctx->nc_groups.size();
@@ -1075,6 +1092,37 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
                          c->d_nc_idx, c->d_nc_off, (uint64_t)jobs[j].n, ws.nc_sums + 5 * nc0[j]);
       ZP_HIP(hipGetLastError());
     }
+  }
+  const bool host_fin = bopts.host_finalize || jobs[0].ctx->opts.host_finalize;
+  if (!host_fin) {
+    // ---- device finalisation: window combine, exceptional points, wire form
+    std::vector<FinJobDev> fin(count);
+    for (size_t j = 0; j < count; j++) {
+      fin[j].win0 = jd[j].win0;
+      fin[j].W = jd[j].W;
+      fin[j].nc0 = (uint32_t)nc0[j];
+      fin[j].ng = (uint32_t)jobs[j].ctx->nc_groups.size();
+      fin[j].mult = jobs[j].ctx->d_nc_mult;
+    }
+    if (!ws.d_consts) {
+      ZP_TRY(grow(ws.d_consts, ws.cap_consts, 1));
+      ZP_HIP(hipMemcpy(ws.d_consts, &KC, sizeof(FConsts), hipMemcpyHostToDevice));
+    }
+    ZP_TRY(grow(ws.fin, ws.cap_fin, count));
+    ZP_TRY(grow(ws.d_out, ws.cap_out, 18 * count));
+    ZP_HIP(hipMemcpyAsync(ws.fin, fin.data(), count * sizeof(FinJobDev), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_msm_finalize, dim3((unsigned)count), dim3(4), 0, stream, ws.fin, ws.wins, ws.winsum,
+                       ws.nc_sums, ws.d_out, ws.d_consts);
+    ZP_HIP(hipGetLastError());
+    ws.h_out.resize(18 * count);
+    ZP_HIP(hipMemcpyAsync(ws.h_out.data(), ws.d_out, count * 18 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    ZP_HIP(hipStreamSynchronize(stream));
+    for (size_t j = 0; j < count; j++) memcpy(jobs[j].out, ws.h_out.data() + 18 * j, sizeof(g1_projective_t));
+    return ZP_OK;
+  }
+  ws.h_winsum.resize(Wt);
+  ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
+  if (nc0[count]) {
     ws.h_nc_sums.resize(5 * nc0[count]);
     ZP_HIP(hipMemcpyAsync(ws.h_nc_sums.data(), ws.nc_sums, 5 * nc0[count] * sizeof(uint64_t), hipMemcpyDeviceToHost,
                           stream));
@@ -1252,6 +1300,35 @@ static int init_points(zp_msm_context *ctx, const g1_affine_t *points, bool on_d
     ZP_HIP(hipMalloc(&ctx->d_nc_off, off.size() * sizeof(uint32_t)));
     ZP_HIP(hipMemcpy(ctx->d_nc_idx, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     ZP_HIP(hipMemcpy(ctx->d_nc_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // P, 2P, 3P of each group for k_msm_finalize.  A curve point without a
+    // twisted-Edwards image has order 2 or 4, so S P = (S mod 4) P; anything
+    // else is not a curve point and is rejected.
+    std::vector<NcMultDev> mult(ctx->nc_groups.size());
+    for (size_t g = 0; g < ctx->nc_groups.size(); g++) {
+      const NcGroup &grp = ctx->nc_groups[g];
+      Jac m;
+      m.X = grp.x;
+      m.Y = grp.y;
+      m.Z = K.one;
+      for (int k = 0; k < 4; k++) {
+        if (k > 0) jac_add_aff(m, grp.x, grp.y, K);  // m = (k + 1) P
+        if (k == 3) break;
+        NcMultDev &d = mult[g];
+        d.inf[k] = f_is_zero_canon(m.Z);
+        d.x[k] = f_zero();
+        d.y[k] = f_zero();
+        if (!d.inf[k]) {
+          const F zi = f_inv(m.Z, K.one), zi2 = f_sqr(zi);
+          d.x[k] = f_canonical(f_mul(m.X, zi2));
+          d.y[k] = f_canonical(f_mul(m.Y, f_mul(zi2, zi)));
+        }
+      }
+      ZP_CHECK(f_is_zero_canon(m.Z), ZP_ERR_INVALID_ARG,
+               "msm_init: point %u has no twisted-Edwards image and is not a curve point (4 P != 0)",
+               (unsigned)grp.idx[0]);
+    }
+    ZP_HIP(hipMalloc(&ctx->d_nc_mult, mult.size() * sizeof(NcMultDev)));
+    ZP_HIP(hipMemcpy(ctx->d_nc_mult, mult.data(), mult.size() * sizeof(NcMultDev), hipMemcpyHostToDevice));
   }
   return ZP_OK;
 }
@@ -1424,6 +1501,7 @@ extern "C" void msm_free(zp_msm_context *ctx) {
   if (ctx->d_conv) (void)hipFree(ctx->d_conv);
   if (ctx->d_nc_idx) (void)hipFree(ctx->d_nc_idx);
   if (ctx->d_nc_off) (void)hipFree(ctx->d_nc_off);
+  if (ctx->d_nc_mult) (void)hipFree(ctx->d_nc_mult);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -430,3 +430,92 @@ def test_host_scalars_pinned_upload_batches():
     finally:
         dk.free()
         dp.free()
+
+
+def _order2_points():
+    """The three curve points (x, 0) with x^3 = -1 (order 2), as wire-form affine records."""
+    w = next(pow(b, (Q - 1) // 3, Q) for b in range(2, 100) if pow(b, (Q - 1) // 3, Q) != 1)
+    pts = np.zeros(3, dtype=zmsm.G1_AFFINE_DTYPE)
+    for i, x in enumerate((Q - 1, (-w) % Q, (-w * w) % Q)):
+        assert (x ** 3 + 1) % Q == 0
+        pts[i]["x"] = oracle.to_mont(x)
+        pts[i]["y"] = oracle.to_mont(0)
+    return pts
+
+
+def test_all_five_exceptional_points_device_vs_host_finalize(msm_golden):
+    """All five curve points without a twisted-Edwards image (the three of
+    order 2, the two of order 4 from msm_unit_tests.rs:30-48) repeated in a
+    table, with scalar sums covering every residue mod 4: the device
+    finalisation (S mod 4 multiples) equals the host finalisation (full
+    double-and-add over the exact sums, option finalize=host) and the oracle
+    (the reference's per-point fallback restated), for several jobs in one batch."""
+    _say("msm five exceptional points, device vs host finalisation")
+    names = {c["name"]: c for c in msm_golden["msm_cases"]}
+    s1, s2 = [oracle.affine_from_json(p) for p in names["unit_intermediate_infinity_1"]["points"][:2]]
+    special = list(_order2_points()) + [s1, s2]
+    rng = random.Random(55)
+    n = 2000
+    pts = zmsm.generate_points(_words([rng.randrange(1, R) for _ in range(n)], 4))
+    picks = rng.sample(range(n), 500)
+    for t, i in enumerate(picks):
+        pts[i] = special[t % 5]
+    ws = zmsm.MsmWorkspace()
+    bufs = []
+    try:
+        with zmsm.MsmContext(pts) as ctx:
+            assert ctx.num_nonconvertible() == 500
+            jobs, want = [], []
+            for j in range(8):
+                sc = _scalars(rng, n)
+                if j < 4:  # force the sum of each group's scalars to residue j mod 4
+                    for g in range(5):
+                        idx = [i for t, i in enumerate(picks) if t % 5 == g]
+                        sc[idx] = 0
+                        sc[idx[0]] = oracle.int_to_words(j, 4)
+                m = n if j % 2 == 0 else rng.randrange(1, n)
+                b = zmsm.DeviceBuffer(32 * n).upload(sc)
+                bufs.append(b)
+                jobs.append((ctx, m, b.ptr))
+                want.append(oracle.msm(pts[:m], sc[:m], 1, threads=8)[0])
+            dev = ws.run(jobs)
+            host = ws.run(jobs, "finalize=host")
+            for j in range(len(jobs)):
+                assert dev[j].tobytes() == want[j].tobytes(), j
+                assert host[j].tobytes() == want[j].tobytes(), j
+    finally:
+        ws.close()
+        for b in bufs:
+            b.free()
+
+
+def test_off_curve_point_without_image_rejected():
+    """A point (x, 1) with Montgomery u = -1 (x = alpha - 1/s) is not on the
+    curve: it has no twisted-Edwards image and 4P (by the group-law formulas)
+    is not the identity, so msm_init rejects it (the finalisation relies on
+    S P = (S mod 4) P for such points).  (A y = 0 point doubles to the
+    identity under the formulas whether or not it lies on the curve, so it
+    is taken as of order 2.)"""
+    from hardcaml_zprize_amd._lib import ZpError
+
+    s = 0x19D47D415B5FF60A87A8B7BBAB25EB6427DD58CA38E47030EFD1E6310AC7BF3079221BF2B4BD72C5106E9E70FCC6156
+    x = (-1 - pow(s, Q - 2, Q)) % Q
+    assert (1 - x ** 3 - 1) % Q != 0
+    pts = zmsm.generate_points(_words([3, 5], 4))
+    bad = np.zeros(1, dtype=zmsm.G1_AFFINE_DTYPE)
+    bad[0]["x"] = oracle.to_mont(x)
+    bad[0]["y"] = oracle.to_mont(1)
+    with pytest.raises(ZpError, match="not a curve point"):
+        zmsm.MsmContext(np.concatenate([pts, bad]))
+
+
+def test_finalize_option_rejected():
+    from hardcaml_zprize_amd._lib import ZpError
+
+    ws = zmsm.MsmWorkspace()
+    try:
+        with zmsm.MsmContext(zmsm.generate_points(_words([1, 2], 4))) as ctx:
+            with pytest.raises(ZpError, match="finalize"):
+                ws.run([(ctx, 0, None)], "finalize=cpu")
+    finally:
+        ws.close()
