@@ -361,89 +361,93 @@ def bench_msm_table(args, probe, torch, options, steps=None, multi=True):
                          "basis": "140 N Fq muls x 288 v_mad_u64_u32 (reference algorithm) / resident-scalar MSM time"}}
 
 
-def bench_batched(args):
+def bench_batched(args, torch):
     """256 NTTs (log2 n ~ U[10,20]) + 64 MSMs (log2 N ~ U[12,22]) over one 2^22 table:
-    plain one-at-a-time calls vs one batch (graph-replayed NTT groups + one MSM launch sequence)."""
-    import threading
-
-    import numpy as np
-
+    plain one-at-a-time calls vs the whole batch as ONE HIP graph launch
+    (zp_mixed_batch: NTT groups and the MSM launch sequence as parallel
+    branches), per-job MSM windows from the committed autotuner table."""
+    from hardcaml_zprize_amd import autotune
     from hardcaml_zprize_amd import msm as zmsm
     from hardcaml_zprize_amd import ntt as zntt
-    from hardcaml_zprize_amd._lib import call
-
     from hardcaml_zprize_amd import workload
+    from hardcaml_zprize_amd._lib import call
+    from hardcaml_zprize_amd.batch import MixedBatch
 
     rng, ntt_logs, ntt_dirs, msm_logs = workload.batched_jobs(args.seed + 2000)
     ctx, ks, _, _ = workload.generate_table(rng, 22, args.batch_options)
-    P = workload.P
-    ntt_bufs = []
-    for k in ntt_logs:
-        b = zmsm.DeviceBuffer(8 << k).upload(rng.integers(0, P, size=1 << k, dtype=np.uint64))
-        ntt_bufs.append(b)
+    ntt_in = [workload.rand_ntt_input(rng, 1 << k) for k in ntt_logs]
+    ntt_bufs = [zmsm.DeviceBuffer(8 << k) for k in ntt_logs]
     msm_sc = [workload.rand_scalars(rng, 1 << k) for k in msm_logs]
     msm_bufs = [zmsm.DeviceBuffer(32 << k).upload(sc) for k, sc in zip(msm_logs, msm_sc)]
-    plans = {}
-    for k in set(ntt_logs):
-        plans[k] = zntt.NttGpuDriver(k, num_buffers=1)
+    plans = {k: zntt.NttGpuDriver(k, num_buffers=1) for k in set(ntt_logs)}
+    table = autotune.load()
+    plain_ws = zmsm.MsmWorkspace()
+    s1 = ctypes.c_void_p()
+    call("zp_stream_create", ctypes.byref(s1))
+
+    def reset_inputs():
+        for b, x in zip(ntt_bufs, ntt_in):
+            b.upload(x)
+
+    def ntt_outputs():
+        return [b.download(np.empty(1 << k, dtype=np.uint64)) for b, k in zip(ntt_bufs, ntt_logs)]
 
     def plain():
         lat = []
+        res = []
         t0 = time.perf_counter()
-        order = [("n", i) for i in range(256)] + [("m", i) for i in range(64)]
-        for kind, i in order:
-            if kind == "n":
-                plans[ntt_logs[i]].inplace_device(ntt_bufs[i].ptr, ntt_dirs[i])
-                call("zp_device_synchronize")
-            else:
-                ctx.mult_prefix_device(1 << msm_logs[i], msm_bufs[i].ptr)
+        for i in range(256):
+            plans[ntt_logs[i]].inplace_device(ntt_bufs[i].ptr, ntt_dirs[i])
+            call("zp_device_synchronize")
             lat.append(time.perf_counter() - t0)
-        return time.perf_counter() - t0, lat
+        for i in range(64):  # one job per call, with the same tuned window the batch uses
+            res.append(plain_ws.run([(ctx, 1 << msm_logs[i], msm_bufs[i].ptr, table.get(msm_logs[i], 0))])[0])
+            lat.append(time.perf_counter() - t0)
+        return time.perf_counter() - t0, lat, res
 
-    nb = zntt.NttBatch([b.ptr for b in ntt_bufs], ntt_logs, ntt_dirs)
-    ws = zmsm.MsmWorkspace()
-    s1, s2 = ctypes.c_void_p(), ctypes.c_void_p()
-    call("zp_stream_create", ctypes.byref(s1))
-    call("zp_stream_create", ctypes.byref(s2))
-    msm_specs = [(ctx, 1 << k, b.ptr) for k, b in zip(msm_logs, msm_bufs)]
+    t0 = time.perf_counter()
+    mb = MixedBatch([(b.ptr, k, d) for b, k, d in zip(ntt_bufs, ntt_logs, ntt_dirs)],
+                    [(ctx, 1 << k, b.ptr) for k, b in zip(msm_logs, msm_bufs)], table=table)
+    create_s = time.perf_counter() - t0
 
     def batched():
         t0 = time.perf_counter()
-        res = {}
-
-        def run_msm():
-            res["msm"] = ws.run(msm_specs, stream=s2.value)
-            res["t_msm"] = time.perf_counter() - t0
-
-        th = threading.Thread(target=run_msm)
-        th.start()
-        nb.launch(use_graph=True, stream=s1.value)
-        call("zp_stream_synchronize", s1)
-        t_ntt = time.perf_counter() - t0
-        th.join()
+        mb.launch(s1.value)
+        res = mb.wait(s1.value)
         total = time.perf_counter() - t0
-        return total, [t_ntt] * 256 + [res["t_msm"]] * 64, res["msm"]
+        return total, [total] * 320, res
 
-    plain()  # warm-up (tables, plans, workspaces)
-    batched()
+    # correctness: every NTT output of the graph equals the plain call's, every
+    # MSM result equals the plain call's and (sum s_i k_i mod r) G
+    reset_inputs()
+    _, _, plain_msm = plain()
+    want_ntt = ntt_outputs()
+    for rep in range(2):  # graph replays, eager work (re-uploads, reads) between them
+        reset_inputs()
+        _, _, got_msm = batched()
+        got_ntt = ntt_outputs()
+        if not all(np.array_equal(a, b) for a, b in zip(want_ntt, got_ntt)):
+            raise RuntimeError(f"batched graph replay {rep}: an NTT differs from its one-at-a-time result")
+        if not all(a.tobytes() == b.tobytes() for a, b in zip(plain_msm, got_msm)):
+            raise RuntimeError(f"batched graph replay {rep}: an MSM differs from its one-at-a-time result")
+    for i, k in enumerate(msm_logs):
+        n = 1 << k
+        if result_words(plain_msm[i]) != closed_form_wire(dot_mod_r_device(torch, msm_sc[i], ks[:n])):
+            raise RuntimeError(f"batched MSM job {i} (2^{k}) differs from (sum s_i k_i mod r) G")
     reps = 3
     tp = [plain() for _ in range(reps)]
     tb = [batched() for _ in range(reps)]
-    # invariance: each MSM result in the batch equals the job run alone
-    alone = [ctx.mult_prefix_device(n, p) for _, n, p in msm_specs]
-    same = all(a.tobytes() == b.tobytes() for a, b in zip(alone, tb[-1][2]))
-    plain_t = float(np.median([t for t, _ in tp]))
+    plain_t = float(np.median([t for t, _, _ in tp]))
     batch_t = float(np.median([t for t, _, _ in tb]))
     lat_p = np.array(tp[len(tp) // 2][1])
     lat_b = np.array(tb[len(tb) // 2][1])
-    nb.close()
-    ws.close()
+    mb.close()
+    plain_ws.close()
     for p in plans.values():
         p.close()
     for b in ntt_bufs + msm_bufs:
         b.free()
     call("zp_stream_destroy", s1)
-    call("zp_stream_destroy", s2)
     ctx.close()
     jobs = 320
     return {"jobs": jobs, "plain_jobs_per_s": jobs / plain_t, "batched_jobs_per_s": jobs / batch_t,
@@ -451,9 +455,14 @@ def bench_batched(args):
             "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3), "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
             "batched_p50_ms": float(np.percentile(lat_b, 50) * 1e3),
             "batched_p99_ms": float(np.percentile(lat_b, 99) * 1e3),
-            "msm_results_equal_alone": bool(same),
-            "note": "NTT inputs resident, transformed in place each repetition; MSM scalars resident; "
-                    "latency measured from batch start"}
+            "graph": True, "graph_create_s": create_s,
+            "autotune_table": os.path.basename(autotune.DEFAULT_PATH) if table else None,
+            "tuned_windows": {str(k): v for k, v in sorted(table.items())},
+            "table_options": args.batch_options,
+            "checked": "graph replays equal the one-at-a-time results (NTT vectors and MSM bytes); every MSM "
+                       "equals (sum s_i k_i mod r) G",
+            "note": "inputs resident; NTTs in place; the batch's latency is its completion time; plain "
+                    "latency is each job's completion time from the start of the sequence"}
 
 
 def bench_served(args):
@@ -754,7 +763,7 @@ def main():
         out["msm"] = bench_msm(args, probe, torch)
         log(f"MSM 2^{args.msm_log_n}: {out['msm']['ms']:.1f} ms (resident scalars)")
     if not args.skip_batch:
-        out["batched"] = bench_batched(args)
+        out["batched"] = bench_batched(args, torch)
         log(f"batched: {out['batched']}")
     if not args.skip_serve:
         out["served"] = bench_served(args)

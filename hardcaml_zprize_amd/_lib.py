@@ -53,6 +53,10 @@ SIGNATURES = {
     "zp_msm_workspace_create": (_i32, [ctypes.POINTER(_vp)]),
     "zp_msm_workspace_destroy": (_i32, [_vp]),
     "zp_msm_batch_device": (_i32, [_vp, _vp, _sz, ctypes.c_char_p, _sz, _vp]),
+    "zp_mixed_batch_create": (_i32, [_vp, _vp, _vp, _sz, _vp, _sz, ctypes.c_char_p, _sz, ctypes.POINTER(_vp)]),
+    "zp_mixed_batch_launch": (_i32, [_vp, _vp]),
+    "zp_mixed_batch_wait": (_i32, [_vp, _vp]),
+    "zp_mixed_batch_destroy": (_i32, [_vp]),
     "zp_g1_mul_generator_device": (_i32, [_vp, _vp, _sz, _vp]),
     "zp_debug_fq_mul": (_i32, [_vp, _vp, _vp, _sz]),
     "zp_debug_msm_digits": (_i32, [ctypes.c_char_p, _sz, _vp, _sz, _vp, ctypes.POINTER(_i32)]),
@@ -70,6 +74,10 @@ SIGNATURES = {
     "zp_memcpy_d2h_async": (_i32, [_vp, _vp, _sz, _vp]),
     "zp_host_alloc": (_i32, [ctypes.POINTER(_vp), _sz]),
     "zp_host_free": (_i32, [_vp]),
+    "zp_transfer_create": (_i32, [ctypes.POINTER(_vp)]),
+    "zp_transfer_destroy": (_i32, [_vp]),
+    "zp_transfer_h2d": (_i32, [_vp, _vp, _vp, _sz]),
+    "zp_transfer_d2h": (_i32, [_vp, _vp, _vp, _sz, _vp]),
     "zp_device_memory_info": (_i32, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
 }
 

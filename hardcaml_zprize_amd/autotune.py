@@ -5,7 +5,7 @@ default policy on the GPU and keeps the fastest; the result is a small JSON
 table that `MsmWorkspace.run` jobs (per-job `window`) and the serving loop
 consult.  `ZP_NTT_TILE` (4 or 8) selects the 4096-point NTT kernels' tile.
 
-    python -m hardcaml_zprize_amd.autotune --min-log 12 --max-log 22 --out autotune.json
+    python -m hardcaml_zprize_amd.autotune --min-log 10 --max-log 22 --options precompute=auto
 """
 import argparse
 import json
@@ -37,6 +37,8 @@ def window_for(log_n, table=None):
 
 def tune_msm(ctx, d_scalars, log_sizes, candidates=None, reps=3):
     """ctx: MsmContext with >= 2^max(log_sizes) points; d_scalars: device scalars.
+    Candidates per size class: 0 (the library's own choice, which includes
+    the window-shifted table when ctx has one) and the default width +- 2.
     Returns {log_n: (best_window, {window: seconds})}."""
     from .msm import MsmWorkspace
 
@@ -45,7 +47,7 @@ def tune_msm(ctx, d_scalars, log_sizes, candidates=None, reps=3):
     try:
         for lg in log_sizes:
             base = default_window(lg)
-            cands = candidates or sorted({min(20, max(8, base + d)) for d in (-2, -1, 0, 1, 2)})
+            cands = candidates or [0] + sorted({min(20, max(8, base + d)) for d in (-2, -1, 0, 1, 2)})
             times = {}
             for w in cands:
                 ws.run([(ctx, 1 << lg, d_scalars, w)])  # warm-up (workspace growth)
@@ -62,34 +64,30 @@ def tune_msm(ctx, d_scalars, log_sizes, candidates=None, reps=3):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--min-log", type=int, default=12)
+    ap.add_argument("--min-log", type=int, default=10)
     ap.add_argument("--max-log", type=int, default=22)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--options", default="precompute=auto",
+                    help="msm_init options of the table tuned on (the batched form's shared table)")
     ap.add_argument("--out", default=DEFAULT_PATH)
     args = ap.parse_args()
     from . import msm as zmsm
+    from . import workload
 
     rng = np.random.default_rng(7)
     n = 1 << args.max_log
-    ks = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    ks[:, 3] &= np.uint64((1 << 60) - 1)
-    dk = zmsm.DeviceBuffer(32 * n).upload(ks)
-    dp = zmsm.DeviceBuffer(104 * n)
-    zmsm.generate_points_device(dk.ptr, dp.ptr, n)
-    ctx = zmsm.MsmContext((dp.ptr, n))
-    dp.free()
-    sc = rng.integers(0, 2**64, size=(n, 4), dtype=np.uint64)
-    sc[:, 3] &= np.uint64((1 << 60) - 1)
-    ds = zmsm.DeviceBuffer(32 * n).upload(sc)
+    ctx, _, _, _ = workload.generate_table(rng, args.max_log, args.options, keep_ks=False)
+    ds = zmsm.DeviceBuffer(32 * n).upload(workload.rand_scalars(rng, n))
     res = tune_msm(ctx, ds.ptr, range(args.min_log, args.max_log + 1), reps=args.reps)
     table = {str(k): v[0] for k, v in res.items()}
     detail = {str(k): {str(w): t for w, t in v[1].items()} for k, v in res.items()}
     with open(args.out, "w") as f:
-        json.dump({"msm_window": table, "seconds": detail}, f, indent=1)
-    print(json.dumps({"msm_window": table}))
+        json.dump({"msm_window": table, "seconds": detail, "table_options": args.options,
+                   "table_points": n, "note": "window 0 = the library's own choice (size-class default or the "
+                                              "window-shifted table)"}, f, indent=1)
+    print(json.dumps({"msm_window": table}), flush=True)
     ctx.close()
     ds.free()
-    dk.free()
 
 
 if __name__ == "__main__":

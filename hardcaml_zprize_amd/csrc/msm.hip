@@ -551,6 +551,55 @@ struct HostUploader {
     ZP_CHECK(!failed, ZP_ERR_HIP, "scalar upload through pinned staging failed");
     return ZP_OK;
   }
+
+  // Device -> pageable host, the reverse pipeline: each lane DMAs its slice
+  // slab by slab into its pinned slabs (after `after` completes on its
+  // stream) while its thread copies the previous slab out.
+  int download(void *dst, const void *src, size_t bytes, hipStream_t after) {
+    if (bytes == 0) return ZP_OK;
+    if (after) ZP_HIP(hipStreamSynchronize(after));
+    if (bytes < 4 * kSlab) {
+      ZP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+      return ZP_OK;
+    }
+    ZP_TRY(init());
+    int dev = 0;
+    ZP_HIP(hipGetDevice(&dev));
+    const size_t per = ((bytes + kLanes - 1) / kLanes + 4095) & ~size_t(4095);
+    std::atomic<int> failed{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < kLanes; t++) {
+      const size_t b0 = std::min(bytes, per * t), b1 = std::min(bytes, b0 + per);
+      if (b0 >= b1) break;
+      th.emplace_back([&, t, b0, b1] {
+        Lane &l = lanes[t];
+        bool ok = hipSetDevice(dev) == hipSuccess;
+        auto issue = [&](size_t o, int k) {
+          const size_t len = std::min(kSlab, b1 - o);
+          return hipMemcpyAsync(l.slab[k], (const char *)src + o, len, hipMemcpyDeviceToHost, l.s) == hipSuccess &&
+                 hipEventRecord(l.sent[k], l.s) == hipSuccess;
+        };
+        int k = 0;
+        if (ok) ok = issue(b0, 0);
+        for (size_t o = b0; ok && o < b1; o += kSlab, k ^= 1) {
+          if (o + kSlab < b1) ok = issue(o + kSlab, k ^ 1);  // next slab in flight while this one is copied
+          if (ok) ok = hipEventSynchronize(l.sent[k]) == hipSuccess;
+          if (ok) memcpy((char *)dst + o, l.slab[k], std::min(kSlab, b1 - o));
+        }
+        if (hipStreamSynchronize(l.s) != hipSuccess) ok = false;
+        if (!ok) failed = 1;
+      });
+    }
+    for (auto &x : th) x.join();
+    ZP_CHECK(!failed, ZP_ERR_HIP, "download through pinned staging failed");
+    return ZP_OK;
+  }
+};
+
+// Stand-alone host <-> device staging for callers with large pageable
+// arrays (the serving loop's job inputs and NTT results).
+struct zp_transfer {
+  HostUploader u;
 };
 
 struct zp_msm_context {
@@ -728,18 +777,40 @@ struct HostJob {
   int window = 0;  // per-job window override (autotuner), 0 = none
 };
 
-static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, const MsmOptions &bopts,
-                     hipStream_t stream) {
-  if (count == 0) return ZP_OK;
-  // ---- plan
-  std::vector<HostLayout> layouts(count);
-  std::vector<MsmJobDev> jd(count);
+// Everything the launch sequence of one batch needs, decided on the host.
+struct BatchPlan {
+  std::vector<HostJob> jobs;
+  std::vector<HostLayout> layouts;
+  std::vector<MsmJobDev> jd;
   std::vector<WinDev> wins;
-  std::vector<uint32_t> pt0(count), b0(count), gstart;
+  std::vector<uint32_t> pt0, b0, gstart, win_g0;
   std::vector<SegDev> segs;
-  uint64_t K = 0, entries = 0, points = 0, counters = 0, n_max = 0;
-  uint32_t max_nb = 0;
-  bool any_pre = false;
+  std::vector<size_t> nc0;
+  std::vector<FinJobDev> fin;
+  uint64_t K = 0, entries = 0, points = 0, counters = 0, n_max = 0, GT = 0, max_units = 0;
+  uint32_t max_nb = 0, max_lbits = 0, max_groups = 1, max_chunks = 1, Wt = 0, log_u = 0, log_g = 0, gs = 0, S = 1;
+  bool any_pre = false, sorted_path = false, msd_path = false, host_fin = false;
+  BatchDev B;
+};
+
+// Plans a batch: layouts, windows, sort groups and work-unit sizes; grows
+// the workspace and uploads the batch's tables on `stream`.  Nothing here is
+// recorded by a graph capture of launch_batch.
+static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, const MsmOptions &bopts,
+                      hipStream_t stream, BatchPlan &P) {
+  P.jobs.assign(jobs, jobs + count);
+  P.layouts.assign(count, HostLayout());
+  P.jd.assign(count, MsmJobDev());
+  P.pt0.assign(count, 0);
+  P.b0.assign(count, 0);
+  std::vector<HostLayout> &layouts = P.layouts;
+  std::vector<MsmJobDev> &jd = P.jd;
+  std::vector<WinDev> &wins = P.wins;
+  std::vector<uint32_t> &pt0 = P.pt0, &b0 = P.b0, &gstart = P.gstart;
+  std::vector<SegDev> &segs = P.segs;
+  uint64_t &K = P.K, &entries = P.entries, &points = P.points, &counters = P.counters, &n_max = P.n_max;
+  uint32_t &max_nb = P.max_nb;
+  bool &any_pre = P.any_pre;
   for (size_t j = 0; j < count; j++) {
     const HostJob &h = jobs[j];
     MsmOptions o = h.ctx->opts;
@@ -851,14 +922,16 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   if (sort == 0) sort = max_nb <= 32768 ? 1 : 2;
   if (sort == 1 && max_nb > 32768) sort = 2;
   if (any_pre) sort = 2;  // a one-window job over shifted tables needs the two-level sort
-  const bool sorted_path = sort == 1, msd_path = sort == 2;
+  P.sorted_path = sort == 1;
+  P.msd_path = sort == 2;
+  const bool sorted_path = P.sorted_path, msd_path = P.msd_path;
   // Two-level split per window: group = bucket >> lbits with about 2^11
   // groups per window (measured at 2^26, window 20: 2^11 groups 157 ms, 2^9
   // 190 ms, 2^7 302 ms: pass B is one block per group), and at most 2^12
   // (pass A's LDS histogram); a few-bucket top window gets one bucket per group.
   const int lb_opt = bopts.sort_lbits >= 0 ? bopts.sort_lbits : jobs[0].ctx->opts.sort_lbits;
-  uint64_t GT = 0;
-  uint32_t max_lbits = 0, max_groups = 1, max_chunks = 1;
+  uint64_t &GT = P.GT;
+  uint32_t &max_lbits = P.max_lbits, &max_groups = P.max_groups, &max_chunks = P.max_chunks;
   for (size_t j = 0; j < count; j++) max_chunks = std::max(max_chunks, jd[j].chunks);
   for (size_t j = 0; j < count; j++) {
     MsmJobDev &d = jd[j];
@@ -895,16 +968,17 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   }
   ZP_CHECK(points < (1ull << 31) && K < (1ull << 31) && entries < (1ull << 32), ZP_ERR_TOO_LARGE,
            "MSM batch too large (%llu points, %llu buckets)", (unsigned long long)points, (unsigned long long)K);
-  const uint32_t Wt = (uint32_t)wins.size();
-  const uint32_t log_u = auto_log_u(entries, bopts.log_u >= 0 ? bopts.log_u : jobs[0].ctx->opts.log_u);
-  const uint32_t log_g = auto_log_g(K, bopts.log_g >= 0 ? bopts.log_g : jobs[0].ctx->opts.log_g);
-  uint32_t gs = 0;
+  const uint32_t Wt = P.Wt = (uint32_t)wins.size();
+  const uint32_t log_u = P.log_u = auto_log_u(entries, bopts.log_u >= 0 ? bopts.log_u : jobs[0].ctx->opts.log_u);
+  const uint32_t log_g = P.log_g = auto_log_g(K, bopts.log_g >= 0 ? bopts.log_g : jobs[0].ctx->opts.log_g);
+  uint32_t &gs = P.gs;
+  gs = 0;
   for (uint32_t w = 0; w < Wt; w++) {
     gstart.push_back(gs);
     gs += (wins[w].nb + (1u << log_g) - 1) >> log_g;
   }
   gstart.push_back(gs);
-  const uint64_t max_units = (entries >> log_u) + K + 1;
+  const uint64_t max_units = P.max_units = (entries >> log_u) + K + 1;
   ZP_CHECK(max_units < (1ull << 32), ZP_ERR_TOO_LARGE, "MSM batch: too many work units");
   // ---- workspace
   ZP_TRY(grow(ws.cnt, ws.cap_K, K + 1));
@@ -946,11 +1020,11 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(hipMemcpyAsync(ws.gstart, gstart.data(), gstart.size() * 4, hipMemcpyHostToDevice, stream));
   if ((sorted_path || msd_path) && !segs.empty())
     ZP_HIP(hipMemcpyAsync(ws.segs, segs.data(), segs.size() * sizeof(SegDev), hipMemcpyHostToDevice, stream));
-  std::vector<uint32_t> win_g0(wins.size());
-  for (size_t w = 0; w < wins.size(); w++) win_g0[w] = wins[w].g0;
+  P.win_g0.resize(wins.size());
+  for (size_t w = 0; w < wins.size(); w++) P.win_g0[w] = wins[w].g0;
   if (msd_path && !wins.empty())
-    ZP_HIP(hipMemcpyAsync(ws.win_g0, win_g0.data(), wins.size() * 4, hipMemcpyHostToDevice, stream));
-  BatchDev B;
+    ZP_HIP(hipMemcpyAsync(ws.win_g0, P.win_g0.data(), wins.size() * 4, hipMemcpyHostToDevice, stream));
+  BatchDev &B = P.B;
   B.jobs = ws.jobs;
   B.job_pt0 = ws.pt0;
   B.job_b0 = ws.b0;
@@ -958,6 +1032,50 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   B.J = (uint32_t)count;
   B.total_points = (uint32_t)points;
   B.K = (uint32_t)K;
+  // reduction slices: windows of many groups (a shifted-table job's one window)
+  uint32_t max_wgroups = 0;
+  for (uint32_t w = 0; w < Wt; w++) max_wgroups = std::max(max_wgroups, gstart[w + 1] - gstart[w]);
+  P.S = max_wgroups > 8192 ? 32 : 1;
+  if (P.S > 1) ZP_TRY(grow(ws.winpart, ws.cap_winpart, (size_t)Wt * P.S));
+  ZP_TRY(grow(ws.umax, ws.cap_umax, 1));
+  // exceptional-point groups, per job
+  P.nc0.assign(count + 1, 0);
+  for (size_t j = 0; j < count; j++) P.nc0[j + 1] = P.nc0[j] + jobs[j].ctx->nc_groups.size();
+  if (P.nc0[count]) ZP_TRY(grow(ws.nc_sums, ws.cap_nc, 5 * P.nc0[count]));
+  P.host_fin = bopts.host_finalize || jobs[0].ctx->opts.host_finalize;
+  // device finalisation: per-job descriptors, constants, results
+  P.fin.resize(count);
+  for (size_t j = 0; j < count; j++) {
+    P.fin[j].win0 = jd[j].win0;
+    P.fin[j].W = jd[j].W;
+    P.fin[j].nc0 = (uint32_t)P.nc0[j];
+    P.fin[j].ng = (uint32_t)jobs[j].ctx->nc_groups.size();
+    P.fin[j].mult = jobs[j].ctx->d_nc_mult;
+  }
+  if (!ws.d_consts) {
+    ZP_TRY(grow(ws.d_consts, ws.cap_consts, 1));
+    ZP_HIP(hipMemcpy(ws.d_consts, &consts(), sizeof(FConsts), hipMemcpyHostToDevice));
+  }
+  ZP_TRY(grow(ws.fin, ws.cap_fin, count));
+  ZP_TRY(grow(ws.d_out, ws.cap_out, 18 * count));
+  ZP_HIP(hipMemcpyAsync(ws.fin, P.fin.data(), count * sizeof(FinJobDev), hipMemcpyHostToDevice, stream));
+  return ZP_OK;
+}
+
+// The launch sequence of a planned batch: kernels and device-to-device work
+// only (no allocation, no host synchronisation, no memset), so a HIP graph
+// can capture it.  With device finalisation the results are copied to
+// h_out (18 words per job; pinned memory when captured).
+static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t stream, uint64_t *h_out) {
+  const size_t count = P.jobs.size();
+  const HostJob *jobs = P.jobs.data();
+  const bool sorted_path = P.sorted_path, msd_path = P.msd_path, any_pre = P.any_pre;
+  const uint64_t K = P.K, points = P.points, counters = P.counters, n_max = P.n_max, GT = P.GT;
+  const uint64_t max_units = P.max_units;
+  const uint32_t max_nb = P.max_nb, max_lbits = P.max_lbits, max_groups = P.max_groups, max_chunks = P.max_chunks;
+  const uint32_t Wt = P.Wt, log_u = P.log_u, log_g = P.log_g, gs = P.gs, S = P.S;
+  const std::vector<SegDev> &segs = P.segs;
+  const BatchDev &B = P.B;
   const FConsts &KC = consts();
   // ---- kernels: counting sort of (bucket, point) entries
   if (sorted_path) {
@@ -1035,7 +1153,6 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
       ZP_HIP(hipGetLastError());
     }
   }
-  ZP_TRY(grow(ws.umax, ws.cap_umax, 1));
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, stream, ws.umax, (uint64_t)1, 0u);
   hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint32_t)K,
                      log_u, ws.nunits, ws.umax);
@@ -1066,11 +1183,7 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(hipGetLastError());
   // Windows of many groups (a shifted-table job's one window) are summed in
   // slices, then the slices per window.
-  uint32_t max_wgroups = 0;
-  for (uint32_t w = 0; w < Wt; w++) max_wgroups = std::max(max_wgroups, gstart[w + 1] - gstart[w]);
-  const uint32_t S = max_wgroups > 8192 ? 32 : 1;
   if (S > 1) {
-    ZP_TRY(grow(ws.winpart, ws.cap_winpart, (size_t)Wt * S));
     hipLaunchKernelGGL(k_window_sum, dim3(Wt, S), dim3(MSM_THREADS), MSM_THREADS * sizeof(Xyzt), stream, ws.groups,
                        ws.gstart, ws.winpart, KC);
     ZP_HIP(hipGetLastError());
@@ -1081,47 +1194,40 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   }
   ZP_HIP(hipGetLastError());
   // exceptional-point scalar sums, per job and group
-  std::vector<size_t> nc0(count + 1, 0);
-  for (size_t j = 0; j < count; j++) nc0[j + 1] = nc0[j] + jobs[j].ctx->nc_groups.size();
-  if (nc0[count]) {
-    ZP_TRY(grow(ws.nc_sums, ws.cap_nc, 5 * nc0[count]));
-    for (size_t j = 0; j < count; j++) {
-      const zp_msm_context *c = jobs[j].ctx;
-      if (c->nc_groups.empty()) continue;
-      hipLaunchKernelGGL(k_nc_sums, dim3((unsigned)c->nc_groups.size()), dim3(256), 0, stream, jobs[j].d_scalars,
-                         c->d_nc_idx, c->d_nc_off, (uint64_t)jobs[j].n, ws.nc_sums + 5 * nc0[j]);
-      ZP_HIP(hipGetLastError());
-    }
-  }
-  const bool host_fin = bopts.host_finalize || jobs[0].ctx->opts.host_finalize;
-  if (!host_fin) {
-    // ---- device finalisation: window combine, exceptional points, wire form
-    std::vector<FinJobDev> fin(count);
-    for (size_t j = 0; j < count; j++) {
-      fin[j].win0 = jd[j].win0;
-      fin[j].W = jd[j].W;
-      fin[j].nc0 = (uint32_t)nc0[j];
-      fin[j].ng = (uint32_t)jobs[j].ctx->nc_groups.size();
-      fin[j].mult = jobs[j].ctx->d_nc_mult;
-    }
-    if (!ws.d_consts) {
-      ZP_TRY(grow(ws.d_consts, ws.cap_consts, 1));
-      ZP_HIP(hipMemcpy(ws.d_consts, &KC, sizeof(FConsts), hipMemcpyHostToDevice));
-    }
-    ZP_TRY(grow(ws.fin, ws.cap_fin, count));
-    ZP_TRY(grow(ws.d_out, ws.cap_out, 18 * count));
-    ZP_HIP(hipMemcpyAsync(ws.fin, fin.data(), count * sizeof(FinJobDev), hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_msm_finalize, dim3((unsigned)count), dim3(4), 0, stream, ws.fin, ws.wins, ws.winsum,
-                       ws.nc_sums, ws.d_out, ws.d_consts);
+  for (size_t j = 0; j < count; j++) {
+    const zp_msm_context *c = jobs[j].ctx;
+    if (c->nc_groups.empty()) continue;
+    hipLaunchKernelGGL(k_nc_sums, dim3((unsigned)c->nc_groups.size()), dim3(256), 0, stream, jobs[j].d_scalars,
+                       c->d_nc_idx, c->d_nc_off, (uint64_t)jobs[j].n, ws.nc_sums + 5 * P.nc0[j]);
     ZP_HIP(hipGetLastError());
+  }
+  if (P.host_fin) return ZP_OK;
+  // device finalisation: window combine, exceptional points, wire form
+  hipLaunchKernelGGL(k_msm_finalize, dim3((unsigned)count), dim3(4), 0, stream, ws.fin, ws.wins, ws.winsum,
+                     ws.nc_sums, ws.d_out, ws.d_consts);
+  ZP_HIP(hipGetLastError());
+  if (h_out)
+    ZP_HIP(hipMemcpyAsync(h_out, ws.d_out, count * 18 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+  return ZP_OK;
+}
+
+// Plans, launches and waits for one batch; writes every job's out.
+static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, const MsmOptions &bopts,
+                     hipStream_t stream) {
+  if (count == 0) return ZP_OK;
+  BatchPlan P;
+  ZP_TRY(plan_batch(ws, jobs, count, bopts, stream, P));
+  if (!P.host_fin) {
     ws.h_out.resize(18 * count);
-    ZP_HIP(hipMemcpyAsync(ws.h_out.data(), ws.d_out, count * 18 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    ZP_TRY(launch_batch(ws, P, stream, ws.h_out.data()));
     ZP_HIP(hipStreamSynchronize(stream));
     for (size_t j = 0; j < count; j++) memcpy(jobs[j].out, ws.h_out.data() + 18 * j, sizeof(g1_projective_t));
     return ZP_OK;
   }
-  ws.h_winsum.resize(Wt);
-  ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
+  ZP_TRY(launch_batch(ws, P, stream, nullptr));
+  const std::vector<size_t> &nc0 = P.nc0;
+  ws.h_winsum.resize(P.Wt);
+  ZP_HIP(hipMemcpyAsync(ws.h_winsum.data(), ws.winsum, P.Wt * sizeof(Xyzt), hipMemcpyDeviceToHost, stream));
   if (nc0[count]) {
     ws.h_nc_sums.resize(5 * nc0[count]);
     ZP_HIP(hipMemcpyAsync(ws.h_nc_sums.data(), ws.nc_sums, 5 * nc0[count] * sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -1130,9 +1236,8 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
   ZP_HIP(hipStreamSynchronize(stream));
   // ---- host finalisation (window combine, conversion, exceptional points)
   auto fin = [&](size_t j) {
-    const HostJob &h = jobs[j];
-    finalize(layouts[j], ws.h_winsum.data() + jd[j].win0, h.ctx->nc_groups, ws.h_nc_sums.data() + 5 * nc0[j],
-             h.out);
+    finalize(P.layouts[j], ws.h_winsum.data() + P.jd[j].win0, jobs[j].ctx->nc_groups,
+             ws.h_nc_sums.data() + 5 * nc0[j], jobs[j].out);
   };
   const size_t nthreads = std::min<size_t>(count, 16);
   if (nthreads <= 1) {
@@ -1613,6 +1718,160 @@ extern "C" int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs,
     hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), J.out, J.window};
   }
   return run_batch(*ws, hj.data(), count, o, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------- transfers --
+extern "C" int zp_transfer_create(zp_transfer **out) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "zp_transfer_create: NULL out");
+  *out = new zp_transfer();
+  return ZP_OK;
+}
+
+extern "C" int zp_transfer_destroy(zp_transfer *t) {
+  if (t) {
+    t->u.release();
+    delete t;
+  }
+  return ZP_OK;
+}
+
+extern "C" int zp_transfer_h2d(zp_transfer *t, void *d_dst, const void *h_src, size_t bytes) {
+  ZP_CHECK(t != nullptr && (bytes == 0 || (d_dst && h_src)), ZP_ERR_INVALID_ARG, "zp_transfer_h2d: NULL argument");
+  return t->u.upload(d_dst, h_src, bytes);
+}
+
+extern "C" int zp_transfer_d2h(zp_transfer *t, void *h_dst, const void *d_src, size_t bytes, void *after_stream) {
+  ZP_CHECK(t != nullptr && (bytes == 0 || (h_dst && d_src)), ZP_ERR_INVALID_ARG, "zp_transfer_d2h: NULL argument");
+  return t->u.download(h_dst, d_src, bytes, (hipStream_t)after_stream);
+}
+
+// ------------------------------------------------------- mixed batch --
+// NTT groups and one MSM batch captured into ONE HIP graph: the NTT launches
+// on the capture stream and the MSM launch sequence on a forked stream (two
+// parallel branches of the graph), the MSM results copied into pinned host
+// memory by the graph's last node.  Planning, allocation and table uploads
+// happen once at creation; each launch replays every kernel from the
+// current contents of the NTT vectors and the scalar buffers.
+struct zp_mixed_batch {
+  zp_ntt_batch *ntt = nullptr;
+  zp_msm_workspace ws;
+  BatchPlan plan;
+  uint64_t *h_out = nullptr;  // pinned, 18 words per MSM job
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int zp_mixed_batch_destroy(zp_mixed_batch *b) {
+  if (!b) return ZP_OK;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->ntt) (void)zp_ntt_batch_destroy(b->ntt);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  b->ws.release();
+  delete b;
+  return ZP_OK;
+}
+
+static int mixed_capture(zp_mixed_batch *b) {
+  hipStream_t cap = nullptr, s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  int st = ZP_OK;
+  auto hip_ok = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && st == ZP_OK) {
+      set_last_error("zp_mixed_batch_create: %s: %s", what, hipGetErrorString(e));
+      st = ZP_ERR_HIP;
+    }
+  };
+  hip_ok(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "stream");
+  hip_ok(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "stream");
+  hip_ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "event");
+  hip_ok(hipEventCreateWithFlags(&join, hipEventDisableTiming), "event");
+  if (st == ZP_OK) {
+    hip_ok(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (st == ZP_OK) {
+      hip_ok(hipEventRecord(fork, cap), "fork");
+      hip_ok(hipStreamWaitEvent(s2, fork, 0), "fork wait");
+      if (st == ZP_OK && b->ntt) st = zp_ntt_batch_launch(b->ntt, 0, cap);
+      if (st == ZP_OK && !b->plan.jobs.empty()) st = launch_batch(b->ws, b->plan, s2, b->h_out);
+      hip_ok(hipEventRecord(join, s2), "join");
+      hip_ok(hipStreamWaitEvent(cap, join, 0), "join wait");
+      hipGraph_t g = nullptr;
+      hip_ok(hipStreamEndCapture(cap, &g), "end capture");
+      b->graph = g;
+    }
+  }
+  if (st == ZP_OK) hip_ok(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "instantiate");
+  if (fork) (void)hipEventDestroy(fork);
+  if (join) (void)hipEventDestroy(join);
+  if (s2) (void)hipStreamDestroy(s2);
+  if (cap) (void)hipStreamDestroy(cap);
+  return st;
+}
+
+extern "C" int zp_mixed_batch_create(uint64_t *const *ntt_ptrs, const unsigned *ntt_log_ns, const int *ntt_dirs,
+                                     size_t ntt_count, const zp_msm_job *msm_jobs, size_t msm_count,
+                                     const char *options, size_t options_len, zp_mixed_batch **out) {
+  ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "zp_mixed_batch_create: out is NULL");
+  *out = nullptr;
+  ZP_CHECK(msm_jobs != nullptr || msm_count == 0, ZP_ERR_INVALID_ARG, "zp_mixed_batch_create: NULL MSM jobs");
+  const MsmOptions o = parse_options(options, options_len);
+  ZP_TRY(validate_options(o, "zp_mixed_batch_create"));
+  ZP_CHECK(!o.host_finalize, ZP_ERR_INVALID_ARG, "zp_mixed_batch_create: finalize=host cannot be captured");
+  std::vector<HostJob> hj(msm_count);
+  for (size_t j = 0; j < msm_count; j++) {
+    const zp_msm_job &J = msm_jobs[j];
+    ZP_CHECK(J.ctx != nullptr && J.out != nullptr, ZP_ERR_INVALID_ARG, "MSM job %zu: NULL context or output", j);
+    ZP_CHECK(J.n <= J.ctx->n, ZP_ERR_TOO_LARGE, "MSM job %zu: n %zu > %zu points", j, J.n, J.ctx->n);
+    ZP_CHECK(J.d_scalars != nullptr || J.n == 0, ZP_ERR_INVALID_ARG, "MSM job %zu: NULL scalars", j);
+    ZP_CHECK(J.window == 0 || (J.window >= 8 && J.window <= 24), ZP_ERR_INVALID_ARG, "MSM job %zu: window %d", j,
+             J.window);
+    ZP_CHECK(!J.ctx->opts.host_finalize, ZP_ERR_INVALID_ARG,
+             "MSM job %zu: its table was loaded with finalize=host, which cannot be captured", j);
+    hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), J.out, J.window};
+  }
+  zp_mixed_batch *b = new zp_mixed_batch();
+  int st = ZP_OK;
+  if (ntt_count) st = zp_ntt_batch_create(ntt_ptrs, ntt_log_ns, ntt_dirs, ntt_count, &b->ntt);
+  if (st == ZP_OK && msm_count) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+      set_last_error("zp_mixed_batch_create: stream creation failed");
+      st = ZP_ERR_HIP;
+    } else {
+      st = plan_batch(b->ws, hj.data(), msm_count, o, s, b->plan);
+      if (hipStreamSynchronize(s) != hipSuccess && st == ZP_OK) {
+        set_last_error("zp_mixed_batch_create: table upload failed");
+        st = ZP_ERR_HIP;
+      }
+      (void)hipStreamDestroy(s);
+    }
+    if (st == ZP_OK && hipHostMalloc((void **)&b->h_out, 18 * sizeof(uint64_t) * msm_count, hipHostMallocDefault) !=
+                           hipSuccess) {
+      set_last_error("zp_mixed_batch_create: pinned result buffer");
+      st = ZP_ERR_OUT_OF_MEMORY;
+    }
+  }
+  if (st == ZP_OK) st = mixed_capture(b);
+  if (st != ZP_OK) {
+    zp_mixed_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return ZP_OK;
+}
+
+extern "C" int zp_mixed_batch_launch(zp_mixed_batch *b, void *stream) {
+  ZP_CHECK(b != nullptr, ZP_ERR_INVALID_ARG, "zp_mixed_batch_launch: NULL batch");
+  ZP_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return ZP_OK;
+}
+
+extern "C" int zp_mixed_batch_wait(zp_mixed_batch *b, void *stream) {
+  ZP_CHECK(b != nullptr, ZP_ERR_INVALID_ARG, "zp_mixed_batch_wait: NULL batch");
+  ZP_HIP(hipStreamSynchronize((hipStream_t)stream));
+  for (size_t j = 0; j < b->plan.jobs.size(); j++)
+    memcpy(b->plan.jobs[j].out, b->h_out + 18 * j, sizeof(g1_projective_t));
+  return ZP_OK;
 }
 
 extern "C" size_t zp_msm_num_points(const zp_msm_context *ctx) { return ctx ? ctx->n : 0; }

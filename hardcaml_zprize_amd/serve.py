@@ -6,7 +6,8 @@
     repeatedly takes the waiting jobs, in arrival order, that fit the free HBM
     budget and an estimated amount of work per batch (`max_batch_s`, so that
     results stream out instead of finishing together), and runs them as ONE
-    batch (grouped NTT launches + one MSM launch sequence); requests that
+    batch (the NTTs on the worker's persistent plans + one MSM launch sequence);
+    requests that
     arrive while other batches are on the GPU are admitted as soon as a worker
     and memory free up, and their work overlaps on the device;
   - the plain form is the same loop with one worker and one job per batch.
@@ -25,7 +26,7 @@ import numpy as np
 from . import autotune
 from ._lib import call, lib
 from .msm import G1_PROJ_DTYPE, MsmWorkspace
-from .ntt import NttBatch
+from .ntt import NttGpuDriver
 
 NTT, MSM = "ntt", "msm"
 
@@ -194,18 +195,22 @@ class GpuServer:
             j._event.set()
 
     def _worker(self):
-        stream = ws = None
+        stream = ws = xfer = None
         try:
             s = ctypes.c_void_p()
             call("zp_stream_create", ctypes.byref(s))
             stream = s.value
             ws = MsmWorkspace()
+            xfer = _Transfer()
         except Exception as e:
             if stream is not None:
                 lib().zp_stream_destroy(ctypes.c_void_p(stream))
+            if ws is not None:
+                ws.close()
             self._fail(e)
             return
         pool = _DevicePool()
+        plans = {}  # log n -> NTT plan (twiddle tables + 4-step scratch), kept for the worker's life
         try:
             while True:
                 taken = self._take()
@@ -216,7 +221,7 @@ class GpuServer:
                 for j in batch:
                     j.t_start = t0
                 try:
-                    self._run_batch(batch, stream, ws, pool)
+                    self._run_batch(batch, stream, ws, pool, plans, xfer)
                 except Exception as e:  # surfaced on the job
                     for j in batch:
                         j.error = e
@@ -229,52 +234,79 @@ class GpuServer:
                     j._event.set()
         finally:
             ws.close()
+            xfer.close()
+            for p in plans.values():
+                p.close()
             pool.release()
             lib().zp_stream_destroy(ctypes.c_void_p(stream))
 
-    def _run_batch(self, batch, stream, ws, pool):
+    def _run_batch(self, batch, stream, ws, pool, plans, xfer):
+        """One batch on the worker's stream: inputs to HBM through the
+        worker's pinned staging, the NTTs on the worker's persistent plans
+        (no allocation per batch), the MSMs as ONE launch sequence, NTT
+        results back through the staging.  The stream is synchronized and the
+        pool buffers returned whatever fails."""
         ntt_jobs = [j for j in batch if j.kind == NTT]
         msm_jobs = [j for j in batch if j.kind == MSM]
         held = []
-        keep = []  # host arrays that async copies read from, alive until the sync below
         sp = ctypes.c_void_p(stream)
-        # inputs to HBM
-        for j in ntt_jobs:
-            nb = 8 << j.log_n
-            p = pool.get(nb)
-            held.append((nb, p))
-            j.result = np.ascontiguousarray(j.data, dtype=np.uint64)
-            call("zp_memcpy_h2d_async", ctypes.c_void_p(p), j.result.ctypes.data_as(ctypes.c_void_p), nb, sp)
-        msm_ptrs = []
-        for j in msm_jobs:
-            nb = 32 * max(len(j.scalars), 1)
-            p = pool.get(nb)
-            held.append((nb, p))
-            sc = np.ascontiguousarray(j.scalars, dtype=np.uint64)
-            keep.append(sc)
-            call("zp_memcpy_h2d_async", ctypes.c_void_p(p), sc.ctypes.data_as(ctypes.c_void_p), sc.nbytes, sp)
-            msm_ptrs.append(p)
-        ntt_ptrs = [p for (nb, p) in held[:len(ntt_jobs)]]
-        b = None
-        if ntt_jobs:
-            b = NttBatch(ntt_ptrs, [j.log_n for j in ntt_jobs], [j.direction for j in ntt_jobs])
-            b.launch(use_graph=False, stream=stream)
-        if msm_jobs:
-            specs = [(self.tables[j.table], len(j.scalars), p,
-                      j.window or self.windows.get(max(len(j.scalars) - 1, 1).bit_length(), 0))
-                     for j, p in zip(msm_jobs, msm_ptrs)]
-            out = ws.run(specs, stream=stream)
-            for j, o in zip(msm_jobs, out):
-                j.result = o
-        for j, p in zip(ntt_jobs, ntt_ptrs):
-            res = np.empty(1 << j.log_n, dtype=np.uint64)
-            call("zp_memcpy_d2h_async", res.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(p), res.nbytes, sp)
-            j.result = res
-        call("zp_stream_synchronize", sp)
-        if b is not None:
-            b.close()
-        for nb, p in held:
-            pool.put(nb, p)
+        try:
+            ntt_ptrs, msm_ptrs = [], []
+            for j in ntt_jobs:
+                nb = 8 << j.log_n
+                p = pool.get(nb)
+                held.append((nb, p))
+                ntt_ptrs.append(p)
+                data = np.ascontiguousarray(j.data, dtype=np.uint64)
+                assert data.nbytes == nb, "NTT job: data length is not 2^log_n"
+                xfer.h2d(p, data)
+            for j in msm_jobs:
+                nb = 32 * max(len(j.scalars), 1)
+                p = pool.get(nb)
+                held.append((nb, p))
+                msm_ptrs.append(p)
+                xfer.h2d(p, np.ascontiguousarray(j.scalars, dtype=np.uint64))
+            for j, p in zip(ntt_jobs, ntt_ptrs):
+                plan = plans.get(j.log_n)
+                if plan is None:
+                    plan = plans[j.log_n] = NttGpuDriver(j.log_n, num_buffers=1)
+                plan.inplace_device(p, j.direction, stream)
+            if msm_jobs:
+                specs = [(self.tables[j.table], len(j.scalars), p,
+                          j.window or self.windows.get(max(len(j.scalars) - 1, 1).bit_length(), 0))
+                         for j, p in zip(msm_jobs, msm_ptrs)]
+                out = ws.run(specs, stream=stream)
+                for j, o in zip(msm_jobs, out):
+                    j.result = o
+            for j, p in zip(ntt_jobs, ntt_ptrs):
+                res = np.empty(1 << j.log_n, dtype=np.uint64)
+                xfer.d2h(res, p, after_stream=stream)
+                j.result = res
+        finally:
+            lib().zp_stream_synchronize(sp)
+            for nb, p in held:
+                pool.put(nb, p)
+
+
+class _Transfer:
+    """zp_transfer: pinned multi-lane staging for large host arrays."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("zp_transfer_create", ctypes.byref(h))
+        self.h = h
+
+    def h2d(self, d_ptr, arr):
+        call("zp_transfer_h2d", self.h, ctypes.c_void_p(d_ptr), arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes)
+
+    def d2h(self, arr, d_ptr, after_stream=None):
+        call("zp_transfer_d2h", self.h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(d_ptr), arr.nbytes,
+             ctypes.c_void_p(after_stream or 0))
+
+    def close(self):
+        if self.h:
+            lib().zp_transfer_destroy(self.h)
+            self.h = None
 
 
 def run_stream(server, jobs, arrivals):

@@ -206,6 +206,25 @@ int zp_msm_workspace_create(zp_msm_workspace **out);
 int zp_msm_workspace_destroy(zp_msm_workspace *ws);
 int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs, size_t count, const char *options,
                         size_t options_len, void *stream);
+/* The batched form as ONE HIP graph: NTT jobs (in place on d_ptrs, as
+ * zp_ntt_batch_create) and MSM jobs (as zp_msm_batch_device; options as
+ * there, e.g. "window=16"; per-job windows from an autotuner table go in
+ * zp_msm_job.window) are planned, allocated and captured once here; the
+ * NTT groups and the MSM launch sequence are parallel branches of the graph.
+ * zp_mixed_batch_launch replays the whole graph on `stream` (asynchronous:
+ * it recomputes every job from the current vectors and scalars);
+ * zp_mixed_batch_wait synchronizes `stream` and writes each MSM job's out.
+ * No reference counterpart (the reference runs one transform / one MSM at a
+ * time on its device, zprize/ntt/host/bench_throughput.cpp:61-107,
+ * zprize/msm_pippenger/host/driver/driver.cpp:495-600). */
+typedef struct zp_mixed_batch zp_mixed_batch;
+int zp_mixed_batch_create(uint64_t *const *ntt_ptrs, const unsigned *ntt_log_ns, const int *ntt_dirs,
+                          size_t ntt_count, const zp_msm_job *msm_jobs, size_t msm_count, const char *options,
+                          size_t options_len, zp_mixed_batch **out);
+int zp_mixed_batch_launch(zp_mixed_batch *b, void *stream);
+int zp_mixed_batch_wait(zp_mixed_batch *b, void *stream);
+int zp_mixed_batch_destroy(zp_mixed_batch *b);
+
 /* Number of loaded points without a twisted-Edwards image (handled by the
  * Weierstrass fallback, driver.cpp:349-372). */
 size_t zp_msm_num_nonconvertible(const zp_msm_context *ctx);
@@ -253,6 +272,17 @@ int zp_stream_synchronize(void *stream);
 int zp_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream);
 int zp_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream);
 int zp_host_alloc(void **ptr, size_t bytes);
+/* Large pageable host arrays <-> HBM through pinned staging: the array is
+ * split over 8 host threads, each copying through its own pair of 16 MiB
+ * pinned slabs while the other slab's DMA runs on its own stream (~2x a
+ * plain pageable copy).  Both calls block until the data has arrived;
+ * zp_transfer_d2h first waits for `after_stream` (may be NULL).  One
+ * transfer at a time per object. */
+typedef struct zp_transfer zp_transfer;
+int zp_transfer_create(zp_transfer **out);
+int zp_transfer_destroy(zp_transfer *t);
+int zp_transfer_h2d(zp_transfer *t, void *d_dst, const void *h_src, size_t bytes);
+int zp_transfer_d2h(zp_transfer *t, void *h_dst, const void *d_src, size_t bytes, void *after_stream);
 int zp_host_free(void *ptr);
 int zp_device_memory_info(size_t *free_bytes, size_t *total_bytes);
 
