@@ -46,6 +46,7 @@ SIGNATURES = {
     "zp_fr_to_montgomery_device": (_i32, [_vp, _u64, _vp]),
     "zp_msm_mult_device": (_i32, [_vp, _vp, _u64, _vp, _vp]),
     "zp_msm_mult_prefix_device": (_i32, [_vp, _sz, _vp, _vp, _vp]),
+    "zp_msm_mult_prefix_noalloc": (_i32, [_vp, _sz, _vp, _vp, _vp]),
     "zp_msm_num_points": (_sz, [_vp]),
     "zp_msm_precompute_width": (_i32, [_vp]),
     "zp_debug_msm_plan": (_i32, [_vp, _sz, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),

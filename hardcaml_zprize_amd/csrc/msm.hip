@@ -448,8 +448,20 @@ struct zp_msm_workspace {
   std::vector<uint64_t> h_out;
   FConsts *d_consts = nullptr;  // the field constants in device memory (k_msm_finalize)
   size_t cap_consts = 0;
+  // Arena mode (zp_msm_mult_prefix_noalloc): every buffer is carved from
+  // [arena, arena + arena_cap) instead of allocated; release() frees nothing.
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
 
   void release() {
+    if (arena) {
+      char *a = arena;
+      const size_t c = arena_cap;
+      *this = zp_msm_workspace();
+      arena = a;
+      arena_cap = c;
+      return;
+    }
     void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
                   groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
                   winpart, umax, fin, d_out, d_consts};
@@ -626,6 +638,22 @@ struct zp_msm_context {
 };
 
 namespace zp {
+
+// A workspace buffer of at least `need` elements: allocated (kept and reused
+// while large enough), or in arena mode carved from the arena, 256-byte
+// aligned, with ZP_ERR_TOO_LARGE when the arena is exhausted.
+template <typename T>
+static int wgrow(zp_msm_workspace &ws, T *&p, size_t &cap, size_t need) {
+  if (!ws.arena) return grow(p, cap, need);
+  const size_t bytes = (std::max<size_t>(need, 1) * sizeof(T) + 255) & ~size_t(255);
+  ZP_CHECK(ws.arena_used + bytes <= ws.arena_cap, ZP_ERR_TOO_LARGE,
+           "MSM scratch: the job needs more than the %zu-byte scratch array (%zu bytes used, %zu more asked)",
+           ws.arena_cap, ws.arena_used, bytes);
+  p = reinterpret_cast<T *>(ws.arena + ws.arena_used);
+  ws.arena_used += bytes;
+  cap = need;
+  return ZP_OK;
+}
 
 static uint32_t auto_log_u(size_t entries, int requested) {
   if (requested >= 0) return (uint32_t)requested;
@@ -981,38 +1009,38 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
   const uint64_t max_units = P.max_units = (entries >> log_u) + K + 1;
   ZP_CHECK(max_units < (1ull << 32), ZP_ERR_TOO_LARGE, "MSM batch: too many work units");
   // ---- workspace
-  ZP_TRY(grow(ws.cnt, ws.cap_K, K + 1));
-  ZP_TRY(grow(ws.off, ws.cap_K2, K + 1));
-  ZP_TRY(grow(ws.cursor, ws.cap_K3, K + 1));
-  ZP_TRY(grow(ws.nunits, ws.cap_K4, K + 1));
-  ZP_TRY(grow(ws.ustart, ws.cap_K5, K + 1));
-  ZP_TRY(grow(ws.buckets, ws.cap_Kb, K + 1));
-  ZP_TRY(grow(ws.vals, ws.cap_entries, std::max<uint64_t>(entries, 1)));
-  ZP_TRY(grow(ws.ubucket, ws.cap_units, max_units));
-  ZP_TRY(grow(ws.partial, ws.cap_units2, max_units));
-  ZP_TRY(grow(ws.groups, ws.cap_groups, (size_t)gs + 1));
-  ZP_TRY(grow(ws.gstart, ws.cap_gs, gstart.size()));
-  ZP_TRY(grow(ws.wins, ws.cap_win, wins.size()));
-  ZP_TRY(grow(ws.winsum, ws.cap_wsum, wins.size()));
-  ZP_TRY(grow(ws.jobs, ws.cap_jobs, count));
-  ZP_TRY(grow(ws.pt0, ws.cap_pt0, count));
-  ZP_TRY(grow(ws.b0, ws.cap_b0, count));
+  ZP_TRY(wgrow(ws, ws.cnt, ws.cap_K, K + 1));
+  ZP_TRY(wgrow(ws, ws.off, ws.cap_K2, K + 1));
+  ZP_TRY(wgrow(ws, ws.cursor, ws.cap_K3, K + 1));
+  ZP_TRY(wgrow(ws, ws.nunits, ws.cap_K4, K + 1));
+  ZP_TRY(wgrow(ws, ws.ustart, ws.cap_K5, K + 1));
+  ZP_TRY(wgrow(ws, ws.buckets, ws.cap_Kb, K + 1));
+  ZP_TRY(wgrow(ws, ws.vals, ws.cap_entries, std::max<uint64_t>(entries, 1)));
+  ZP_TRY(wgrow(ws, ws.ubucket, ws.cap_units, max_units));
+  ZP_TRY(wgrow(ws, ws.partial, ws.cap_units2, max_units));
+  ZP_TRY(wgrow(ws, ws.groups, ws.cap_groups, (size_t)gs + 1));
+  ZP_TRY(wgrow(ws, ws.gstart, ws.cap_gs, gstart.size()));
+  ZP_TRY(wgrow(ws, ws.wins, ws.cap_win, wins.size()));
+  ZP_TRY(wgrow(ws, ws.winsum, ws.cap_wsum, wins.size()));
+  ZP_TRY(wgrow(ws, ws.jobs, ws.cap_jobs, count));
+  ZP_TRY(wgrow(ws, ws.pt0, ws.cap_pt0, count));
+  ZP_TRY(wgrow(ws, ws.b0, ws.cap_b0, count));
   size_t scan_bytes = 0, scan_bytes2 = 0;
   ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
                                  stream));
   if (msd_path) {
-    ZP_TRY(grow(ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1) * (any_pre ? 1 : 2)));
-    ZP_TRY(grow(ws.win_g0, ws.cap_wg0, std::max<size_t>(wins.size(), 1)));
+    ZP_TRY(wgrow(ws, ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1) * (any_pre ? 1 : 2)));
+    ZP_TRY(wgrow(ws, ws.win_g0, ws.cap_wg0, std::max<size_t>(wins.size(), 1)));
   }
   if (sorted_path || msd_path) {
-    ZP_TRY(grow(ws.digits, ws.cap_digits, std::max<uint64_t>(entries, 1)));
-    ZP_TRY(grow(ws.counts, ws.cap_counts, counters + 1));
-    ZP_TRY(grow(ws.scanned, ws.cap_scanned, counters + 1));
-    ZP_TRY(grow(ws.segs, ws.cap_segs, std::max<size_t>(segs.size(), 1)));
+    ZP_TRY(wgrow(ws, ws.digits, ws.cap_digits, std::max<uint64_t>(entries, 1)));
+    ZP_TRY(wgrow(ws, ws.counts, ws.cap_counts, counters + 1));
+    ZP_TRY(wgrow(ws, ws.scanned, ws.cap_scanned, counters + 1));
+    ZP_TRY(wgrow(ws, ws.segs, ws.cap_segs, std::max<size_t>(segs.size(), 1)));
     ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes2, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
                                    rocprim::plus<uint32_t>(), stream));
   }
-  ZP_TRY(grow(ws.scan_tmp, ws.cap_scan, std::max<size_t>(std::max(scan_bytes, scan_bytes2), 16)));
+  ZP_TRY(wgrow(ws, ws.scan_tmp, ws.cap_scan, std::max<size_t>(std::max(scan_bytes, scan_bytes2), 16)));
   ZP_HIP(hipMemcpyAsync(ws.jobs, jd.data(), count * sizeof(MsmJobDev), hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.wins, wins.data(), wins.size() * sizeof(WinDev), hipMemcpyHostToDevice, stream));
   ZP_HIP(hipMemcpyAsync(ws.pt0, pt0.data(), count * 4, hipMemcpyHostToDevice, stream));
@@ -1036,12 +1064,12 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
   uint32_t max_wgroups = 0;
   for (uint32_t w = 0; w < Wt; w++) max_wgroups = std::max(max_wgroups, gstart[w + 1] - gstart[w]);
   P.S = max_wgroups > 8192 ? 32 : 1;
-  if (P.S > 1) ZP_TRY(grow(ws.winpart, ws.cap_winpart, (size_t)Wt * P.S));
-  ZP_TRY(grow(ws.umax, ws.cap_umax, 1));
+  if (P.S > 1) ZP_TRY(wgrow(ws, ws.winpart, ws.cap_winpart, (size_t)Wt * P.S));
+  ZP_TRY(wgrow(ws, ws.umax, ws.cap_umax, 1));
   // exceptional-point groups, per job
   P.nc0.assign(count + 1, 0);
   for (size_t j = 0; j < count; j++) P.nc0[j + 1] = P.nc0[j] + jobs[j].ctx->nc_groups.size();
-  if (P.nc0[count]) ZP_TRY(grow(ws.nc_sums, ws.cap_nc, 5 * P.nc0[count]));
+  if (P.nc0[count]) ZP_TRY(wgrow(ws, ws.nc_sums, ws.cap_nc, 5 * P.nc0[count]));
   P.host_fin = bopts.host_finalize || jobs[0].ctx->opts.host_finalize;
   // device finalisation: per-job descriptors, constants, results
   P.fin.resize(count);
@@ -1053,11 +1081,11 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
     P.fin[j].mult = jobs[j].ctx->d_nc_mult;
   }
   if (!ws.d_consts) {
-    ZP_TRY(grow(ws.d_consts, ws.cap_consts, 1));
+    ZP_TRY(wgrow(ws, ws.d_consts, ws.cap_consts, 1));
     ZP_HIP(hipMemcpy(ws.d_consts, &consts(), sizeof(FConsts), hipMemcpyHostToDevice));
   }
-  ZP_TRY(grow(ws.fin, ws.cap_fin, count));
-  ZP_TRY(grow(ws.d_out, ws.cap_out, 18 * count));
+  ZP_TRY(wgrow(ws, ws.fin, ws.cap_fin, count));
+  ZP_TRY(wgrow(ws, ws.d_out, ws.cap_out, 18 * count));
   ZP_HIP(hipMemcpyAsync(ws.fin, P.fin.data(), count * sizeof(FinJobDev), hipMemcpyHostToDevice, stream));
   return ZP_OK;
 }
@@ -1216,7 +1244,11 @@ static int run_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, co
                      hipStream_t stream) {
   if (count == 0) return ZP_OK;
   BatchPlan P;
-  ZP_TRY(plan_batch(ws, jobs, count, bopts, stream, P));
+  const int rc = plan_batch(ws, jobs, count, bopts, stream, P);
+  if (rc != ZP_OK) {
+    (void)hipStreamSynchronize(stream);  // table uploads already issued read P's vectors
+    return rc;
+  }
   if (!P.host_fin) {
     ws.h_out.resize(18 * count);
     ZP_TRY(launch_batch(ws, P, stream, ws.h_out.data()));
@@ -1718,6 +1750,54 @@ extern "C" int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs,
     hj[j] = HostJob{J.ctx, J.n, reinterpret_cast<const uint64_t *>(J.d_scalars), J.out, J.window};
   }
   return run_batch(*ws, hj.data(), count, o, (hipStream_t)stream);
+}
+
+// ---------------------------------------------- allocation-free entry --
+// Device working memory of zp_msm_mult_prefix_noalloc: one array per GPU,
+// held until the process ends.  Host code never uses the array's name as a
+// device address; it takes the address once per device from
+// hipGetSymbolAddress and passes that pointer on.
+static constexpr size_t ZP_MSM_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) char scratch[ZP_MSM_SCRATCH_BYTES];
+
+static constexpr int ZP_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu[ZP_MAX_DEVICES];
+static char *g_scratch_ptr[ZP_MAX_DEVICES];            // per device, from hipGetSymbolAddress
+static zp_msm_workspace g_noalloc_ws[ZP_MAX_DEVICES];  // arena-mode workspace over that array
+
+extern "C" int zp_msm_mult_prefix_noalloc(zp_msm_context *ctx, size_t n, g1_projective_t *out,
+                                          const biginteger256_t *d_scalars, void *stream) {
+  ZP_CHECK(ctx != nullptr && out != nullptr, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_noalloc: NULL argument");
+  ZP_CHECK(n <= ctx->n, ZP_ERR_TOO_LARGE, "zp_msm_mult_prefix_noalloc: n %zu > %zu points", n, ctx->n);
+  ZP_CHECK(d_scalars != nullptr || n == 0, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_noalloc: NULL scalars");
+  int dev = 0;
+  ZP_HIP(hipGetDevice(&dev));
+  ZP_CHECK(dev >= 0 && dev < ZP_MAX_DEVICES, ZP_ERR_INVALID_ARG, "zp_msm_mult_prefix_noalloc: device %d", dev);
+  // One call at a time per GPU: every call on a device shares its scratch array.
+  std::lock_guard<std::mutex> lock(g_noalloc_mu[dev]);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    ZP_CHECK(e == hipSuccess && ptr != nullptr, ZP_ERR_HIP,
+             "zp_msm_mult_prefix_noalloc: scratch symbol address on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char *>(ptr);
+  }
+  zp_msm_workspace &ws = g_noalloc_ws[dev];
+  ws.arena = g_scratch_ptr[dev];
+  ws.arena_cap = ZP_MSM_SCRATCH_BYTES;
+  ws.release();  // arena mode: forget the last call's buffers, carve again from the start
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  HostJob job{ctx, n, reinterpret_cast<const uint64_t *>(d_scalars), out};
+  BatchPlan P;
+  const int rc = plan_batch(ws, &job, 1, MsmOptions(), s, P);
+  if (rc != ZP_OK) {
+    (void)hipStreamSynchronize(s);  // table uploads already issued read P's vectors
+    return rc;
+  }
+  P.host_fin = false;  // results come from k_msm_finalize straight into out
+  ZP_TRY(launch_batch(ws, P, s, reinterpret_cast<uint64_t *>(out)));
+  ZP_HIP(hipStreamSynchronize(s));
+  return ZP_OK;
 }
 
 // ---------------------------------------------------------- transfers --

@@ -183,6 +183,18 @@ int zp_msm_mult_device(zp_msm_context *ctx, g1_projective_t *out, uint64_t num_b
 int zp_msm_mult_prefix_device(zp_msm_context *ctx, size_t n, g1_projective_t *out, const biginteger256_t *d_scalars,
                               void *stream);
 size_t zp_msm_num_points(const zp_msm_context *ctx);
+/* zp_msm_mult_prefix_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The per-call working memory
+ * (digits, sort entries, counters, bucket and unit partial sums, window sums,
+ * results) lives in a static 64 MiB device array per GPU, held until the
+ * process ends; a job needing more (roughly beyond 2^15 points at the default
+ * window) fails with ZP_ERR_TOO_LARGE and a message giving the sizes.  Calls
+ * on one GPU share that array and are serialized by a per-device lock; the
+ * array keeps the last call's contents.  Blocks until *out is written (host
+ * memory).  Allocates nothing (no hipMalloc/hipFree). */
+int zp_msm_mult_prefix_noalloc(zp_msm_context *ctx, size_t n, g1_projective_t *out,
+                               const biginteger256_t *d_scalars, void *stream);
 /* Width c of the context's window-shifted table ("precompute="), 0 if it has none. */
 int zp_msm_precompute_width(const zp_msm_context *ctx);
 
