@@ -469,8 +469,8 @@ ZP_HD bool f_inv_pornin(const F &a, const FConsts &K, F &out) {
       f0 -= f1 & (int64_t)odd;
       g0 -= g1 & (int64_t)odd;
       a_ >>= 1;
-      f1 <<= 1;
-      g1 <<= 1;
+      f1 *= 2;  // (a left shift of a negative value is undefined before C++20)
+      g1 *= 2;
     }
     uint32_t nA[INV_L], nB[INV_L];
     int64_t ta = inv_lin(A, B, f0, g0, nA), tb = inv_lin(A, B, f1, g1, nB);

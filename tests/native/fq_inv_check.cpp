@@ -3,13 +3,15 @@
 // defining property a * a^-1 = 1, on edge values and 200,000 random field
 // elements; f_inv_pornin must converge within its step bound on every one.
 #include <cstdio>
+#include <cstdlib>
 #include <random>
 
 #include "msm_host.hpp"
 
 using namespace zp;
 
-int main() {
+int main(int argc, char **argv) {
+  const int count = argc > 1 ? atoi(argv[1]) : 20000;
   const FConsts K = make_fconsts();
   std::mt19937_64 rng(377);
   int bad = 0, n = 0;
@@ -56,7 +58,7 @@ int main() {
     }
     check(f_from_u384(h));
   }
-  for (int i = 0; i < 20000; i++) {
+  for (int i = 0; i < count; i++) {
     U384 v;
     for (int j = 0; j < 6; j++) v.w[j] = rng();
     v.w[5] &= (1ull << 57) - 1;  // < 2^377
