@@ -539,9 +539,14 @@ def bench_served(args):
                "served_batches": nb2}
     plain.close()
     served.close()
+    bound = device_bound(specs, tables, pool, ntt_pool)
     for t in tables:
         t.close()
     return {"jobs": njobs, "offered_rate_jobs_per_s": rate, "plain_capacity_jobs_per_s": njobs / cal_s,
+            "device_only_bound_jobs_per_s": bound["jobs_per_s"],
+            "served_over_device_bound": (njobs / served_s) / bound["jobs_per_s"],
+            "plain_over_device_bound": (njobs / plain_s) / bound["jobs_per_s"],
+            "device_only": bound,
             "plain_jobs_per_s": njobs / plain_s, "served_jobs_per_s": njobs / served_s,
             "speedup": plain_s / served_s,
             "plain_latency_ms": {k: v * 1e3 for k, v in serve.percentiles(plain_lat).items()},
@@ -551,6 +556,49 @@ def bench_served(args):
             "below_saturation": low,
             "results_equal_plain": bool(same),
             "note": "host inputs (H2D/D2H inside each job); MSM scalars are slices of one seeded pool"}
+
+
+def device_bound(specs, tables, pool, ntt_pool):
+    """Upper bound of any schedule of the served jobs on this GPU: each job
+    timed alone with its inputs already in HBM (no copies), on the same
+    library paths and tuned windows; jobs/s = jobs / sum of these times."""
+    from hardcaml_zprize_amd import autotune
+    from hardcaml_zprize_amd import msm as zmsm
+    from hardcaml_zprize_amd import ntt as zntt
+    from hardcaml_zprize_amd._lib import call
+
+    windows = autotune.load()
+    d_pool = zmsm.DeviceBuffer(pool.nbytes).upload(pool)
+    d_ntt = zmsm.DeviceBuffer(ntt_pool.nbytes).upload(ntt_pool)
+    ws = zmsm.MsmWorkspace()
+    plans = {}
+    t_msm = t_ntt = 0.0
+    try:
+        for rep in range(2):  # the first pass grows workspaces and builds plans
+            t_msm = t_ntt = 0.0
+            for sp in specs:
+                k = sp["log_n"]
+                if sp["kind"] == "msm":
+                    ptr = d_pool.ptr + 32 * sp["off"]
+                    t0 = time.perf_counter()
+                    ws.run([(tables[sp["table"]], 1 << k, ptr, windows.get(k, 0))])
+                    t_msm += time.perf_counter() - t0
+                else:
+                    if k not in plans:
+                        plans[k] = zntt.NttGpuDriver(k, num_buffers=1)
+                    t0 = time.perf_counter()
+                    plans[k].inplace_device(d_ntt.ptr, sp["direction"])
+                    call("zp_device_synchronize")
+                    t_ntt += time.perf_counter() - t0
+    finally:
+        ws.close()
+        for p in plans.values():
+            p.close()
+        d_pool.free()
+        d_ntt.free()
+    total = t_msm + t_ntt
+    return {"jobs_per_s": len(specs) / total, "msm_s": t_msm, "ntt_s": t_ntt,
+            "basis": "every job alone, inputs resident in HBM, synchronous; sum of times"}
 
 
 def probe_rates():

@@ -1,11 +1,13 @@
 // Host program over the C++ NttGpuDriver mirror, with the modes of the
 // reference's zprize/ntt/host programs:
-//   zp_ntt_host test     (test_random.cpp)    round trip + closed forms; the
+//   zp_ntt_host test     (test_random.cpp)    every case against a CPU transform
+//                        (test_random.cpp:56,73-90), plus the round trip; the
 //                        first case is all ones when T == 1 (test_random.cpp:44-52);
 //                        for --core-type REVERSE the element formula of
 //                        reference.cpp:60-84.
 //   zp_ntt_host bench    (bench_latency.cpp)  latency percentiles through the
-//                        buffer API (bench_latency.cpp:101-117).
+//                        buffer API (bench_latency.cpp:101-117), every run's
+//                        output compared with the CPU transform of the input.
 //   zp_ntt_host throughput (bench_throughput.cpp) every buffer loaded once,
 //                        then E evaluations enqueued round-robin over the
 //                        buffers without waiting, then E waits; reports
@@ -41,6 +43,53 @@ static uint64_t powmod(uint64_t a, uint64_t e) {
   for (; e; e >>= 1, a = mulmod(a, a))
     if (e & 1) r = mulmod(r, a);
   return r;
+}
+
+// CPU transform the test and bench modes check the GPU against, as the
+// reference's programs do with ntt_reference (test_random.cpp:56,
+// bench_latency.cpp:95-98): X_k = sum_j x_j w^(jk) with w = 7^((p-1)/n)
+// (w^-1 and a final n^-1 for the inverse), by bit reversal and radix-2
+// decimation-in-time passes with 128-bit modular products.
+static vec64 cpu_ntt(const vec64 &x, bool inverse) {
+  const size_t n = x.size();
+  int lg = 0;
+  while ((size_t(1) << lg) < n) lg++;
+  vec64 a(x);
+  for (auto &v : a) v %= P;
+  for (size_t i = 1, j = 0; i < n; i++) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    uint64_t w = powmod(7, (P - 1) / len);
+    if (inverse) w = powmod(w, P - 2);
+    for (size_t i = 0; i < n; i += len) {
+      uint64_t t = 1;
+      for (size_t k = 0; k < len / 2; k++, t = mulmod(t, w)) {
+        const uint64_t u = a[i + k], v = mulmod(a[i + k + len / 2], t);
+        a[i + k] = (uint64_t)(((unsigned __int128)u + v) % P);
+        a[i + k + len / 2] = (uint64_t)(((unsigned __int128)u + P - v) % P);
+      }
+    }
+  }
+  if (inverse) {
+    const uint64_t ninv = powmod(n % P, P - 2);
+    for (auto &v : a) v = mulmod(v, ninv);
+  }
+  return a;
+}
+
+// test_random.cpp:73-80: the first mismatching element.
+static bool same_or_report(const vec64 &want, const vec64 &got) {
+  for (size_t i = 0; i < want.size(); i++)
+    if (want[i] != got[i]) {
+      std::cout << "Error: Result mismatch" << std::endl
+                << " i = " << i << " Expected = " << want[i] << " Obtained = " << got[i] << std::endl;
+      return false;
+    }
+  return true;
 }
 
 // evaluate_given.cpp:19-56: exactly n hex values (an 0x prefix is accepted).
@@ -193,7 +242,7 @@ static int run(const Args &a) {
       } else {
         drv.simple_evaluate(fwd.data(), in.data(), n, ZP_NTT_FORWARD);
         drv.simple_evaluate(back.data(), fwd.data(), n, ZP_NTT_INVERSE);
-        ok = back == in;
+        ok = same_or_report(cpu_ntt(in, false), fwd) && back == in;
         if (a.cases == 1 && t == 0) {  // all ones -> n * delta
           ok = ok && fwd[0] == n % P;
           for (uint64_t i = 1; i < n; i++) ok = ok && fwd[i] == 0;
@@ -208,9 +257,14 @@ static int run(const Args &a) {
 
   if (a.mode == "bench") {
     // request / transfer / enqueue / wait / transfer / free, per evaluation
+    // one input, its CPU transform computed once, every evaluation compared
+    // with it outside the timed region (bench_latency.cpp:60-98)
     std::vector<double> lat;
+    for (uint64_t i = 0; i < n; i++) in[i] = rng() % P;
+    std::cout << "Computing reference output.." << std::endl;
+    const vec64 expected = cpu_ntt(in, dir == ZP_NTT_INVERSE);
+    int failed = 0;
     for (uint64_t e = 0; e < a.evals; e++) {
-      for (uint64_t i = 0; i < n; i++) in[i] = rng() % P;
       auto t0 = std::chrono::steady_clock::now();
       auto *b = drv.request_buffer();
       drv.transfer_input_vector_to_user_buffer(b, in.data());
@@ -219,12 +273,16 @@ static int run(const Args &a) {
       drv.transfer_user_buffer_to_output_vector(b, fwd.data());
       drv.free_buffer(b);
       lat.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      if (fwd != expected) {
+        std::cout << "Incorrect result in run  " << e << std::endl;
+        failed = 1;
+      }
     }
     std::sort(lat.begin(), lat.end());
     auto q = [&](double f) { return lat[(size_t)(f * (lat.size() - 1))]; };
     printf("latency (s): min %.6f p25 %.6f p50 %.6f p75 %.6f max %.6f\n", lat.front(), q(0.25), q(0.5), q(0.75),
            lat.back());
-    return 0;
+    return failed;
   }
   if (a.mode == "throughput") {
     const uint64_t nb = drv.max_num_buffers_in_flight();

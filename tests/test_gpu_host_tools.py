@@ -16,6 +16,7 @@ TOOL = os.path.join(ROOT, "hardcaml_zprize_amd", "zp_ntt_host")
 @pytest.mark.parametrize("args", [["test", "--log-n", "12", "--num-test-cases", "1"],
                                   ["test", "--log-n", "16", "--num-test-cases", "3"],
                                   ["bench", "--log-n", "14", "--num-evaluations", "10"],
+                                  ["bench", "--log-n", "18", "--num-evaluations", "5", "--inverse"],
                                   ["throughput", "--log-n", "16", "--num-evaluations", "40"],
                                   ["throughput", "--log-n", "20", "--num-evaluations", "20", "--inverse"]])
 def test_host_tool(args):
@@ -23,8 +24,10 @@ def test_host_tool(args):
     r = subprocess.run([TOOL] + args, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
-    if args[0] == "test":
-        assert "FAILED" not in r.stdout and "OK" in r.stdout
+    if args[0] == "test":  # each case also against the tool's CPU transform (test_random.cpp:56,73-90)
+        assert "FAILED" not in r.stdout and "OK" in r.stdout and "mismatch" not in r.stdout
+    if args[0] == "bench":  # every run compared with the CPU transform (bench_latency.cpp:95-98)
+        assert "Incorrect result" not in r.stdout and "latency" in r.stdout
     if args[0] == "throughput":  # bench_throughput.cpp:94-103 output, every buffer checked
         assert "Incorrect result" not in r.stdout and "NTTs/second" in r.stdout
 
