@@ -256,25 +256,25 @@ def result_words(out):
 
 def dot_mod_r_device(torch, s, k):
     """sum_i s_i k_i mod r for uint64 (n, 4) host arrays, on the GPU with
-    torch float64 matrix products over 16-bit limbs: blocks of 2^20 rows keep
-    every partial sum below 2^52, so each block's 16 x 16 limb-pair sums are
-    exact; blocks are accumulated in int64 (< 2^58)."""
+    torch int64 arithmetic over 16-bit limbs: per block of 2^22 rows, each
+    of the 16 x 16 limb-pair sums is a column sum of products < 2^32, below
+    2^54 (exact); blocks are accumulated as Python integers."""
     from hardcaml_zprize_amd import workload
 
     n = len(s)
-    acc = torch.zeros((16, 16), dtype=torch.int64, device="cuda:0")
-    block = 1 << 20
+    acc = [[0] * 16 for _ in range(16)]
+    block = 1 << 22
     for b0 in range(0, n, block):
-        parts = []
-        for a in (s, k):
-            t = torch.from_numpy(a[b0:b0 + block].view(np.int16)).to("cuda:0")
-            parts.append((t.to(torch.int32) & 0xFFFF).to(torch.float64))
-        acc += (parts[0].T @ parts[1]).to(torch.int64)
-    acc = acc.cpu().numpy()
+        sl = torch.from_numpy(s[b0:b0 + block].view(np.int16)).to("cuda:0").to(torch.int64) & 0xFFFF
+        kl = torch.from_numpy(k[b0:b0 + block].view(np.int16)).to("cuda:0").to(torch.int64) & 0xFFFF
+        sums = torch.stack([(sl[:, a:a + 1] * kl).sum(dim=0) for a in range(16)]).cpu().numpy()
+        for a in range(16):
+            for c in range(16):
+                acc[a][c] += int(sums[a, c])
     total = 0
     for a in range(16):
-        for b in range(16):
-            total += int(acc[a, b]) << (16 * (a + b))
+        for c in range(16):
+            total += acc[a][c] << (16 * (a + c))
     return total % workload.R_ORDER
 
 
