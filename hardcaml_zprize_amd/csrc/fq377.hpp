@@ -41,7 +41,7 @@ struct Packed {
 };
 
 // q in radix 2^28.
-ZP_HD uint32_t q28(int i) {
+__host__ __device__ __forceinline__ constexpr uint32_t q28(int i) {
   switch (i) {
     case 0: return 0x1u;
     case 1: return 0x8c0000u;
@@ -140,6 +140,42 @@ ZP_HD void f_make_kq(uint32_t k, uint32_t (&out)[14]) {
   out[0] = v[0] + (1u << 28);
   for (int i = 1; i < 13; i++) out[i] = v[i] + (1u << 28) - 1;
   out[13] = v[13] - 1 + (c << 28);
+}
+
+// Limb i of K q in the borrow-free form of f_make_kq, as a compile-time
+// constant (the subtractions below then take immediates instead of a table).
+__host__ __device__ constexpr uint32_t kq_limb(uint32_t k, int i) {
+  uint32_t v[14] = {};
+  uint32_t c = 0;
+  for (int j = 0; j < 14; j++) {
+    const uint32_t t = q28(j) * k + c;
+    v[j] = t & F_MASK;
+    c = t >> 28;
+  }
+  return i == 0 ? v[0] + (1u << 28) : i < 13 ? v[i] + (1u << 28) - 1 : v[13] - 1 + (c << 28);
+}
+
+// f_sub_kq / f_sub_nc with K q a compile-time constant.
+template <uint32_t KQ>
+ZP_HD F f_sub_kq_c(const F &a, const F &b) {
+  F r;
+  int32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 13; i++) {
+    int32_t v = (int32_t)(a.l[i] + kq_limb(KQ, i) - b.l[i]) + c;
+    r.l[i] = (uint32_t)v & F_MASK;
+    c = v >> 28;
+  }
+  r.l[13] = (uint32_t)((int32_t)(a.l[13] + kq_limb(KQ, 13) - b.l[13]) + c);
+  return r;
+}
+
+template <uint32_t KQ>
+ZP_HD F f_sub_nc_c(const F &a, const F &b) {
+  F r;
+#pragma unroll
+  for (int i = 0; i < 14; i++) r.l[i] = a.l[i] + kq_limb(KQ, i) - b.l[i];
+  return r;
 }
 
 // Montgomery product a * b / 2^392 mod q (CIOS, lazy carries).

@@ -49,6 +49,16 @@ __device__ __forceinline__ F shfl4(const F &v, int src) {
   return r;
 }
 
+// Every lane's product p; then the 4 products in every lane (lane shuffles:
+// an exchange through LDS with 16-byte accesses and barriers measured 9 %
+// slower per doubling).
+__device__ __forceinline__ void swap4(const F &p, F &r0, F &r1, F &r2, F &r3) {
+  r0 = shfl4(p, 0);
+  r1 = shfl4(p, 1);
+  r2 = shfl4(p, 2);
+  r3 = shfl4(p, 3);
+}
+
 // The operand of lane `sel`, by masks: a select between the structures
 // themselves becomes a select between their addresses, i.e. scratch memory.
 __device__ __forceinline__ F sel4(int sel, const F &a, const F &b, const F &c, const F &d) {
@@ -60,40 +70,35 @@ __device__ __forceinline__ F sel4(int sel, const F &a, const F &b, const F &c, c
   return r;
 }
 
-// a = 2a as ec_dbl, the 8 products spread over the 4 lanes.
-__device__ __forceinline__ void ec_dbl4(Xyzt &a, int sel, const FConsts &K) {
-  const F p = f_sqr(sel4(sel, a.x, a.y, a.z, f_add(a.x, a.y)));
-  const F A = shfl4(p, 0), B = shfl4(p, 1), zz = shfl4(p, 2), S = shfl4(p, 3);
-  const F C = f_add(zz, zz);
-  const F AB = f_add(A, B);
-  const F E = f_sub_kq(S, AB, K.q4);
-  const F G = f_sub_kq(B, A, K.q2);
-  const F Fm = f_sub_kq(G, C, K.q4);
-  const F H = f_sub_kq(f_zero(), AB, K.q4);
-  const F r = f_mul(sel4(sel, E, G, E, Fm), sel4(sel, Fm, H, H, G));
-  a.x = shfl4(r, 0);
-  a.y = shfl4(r, 1);
-  a.t = shfl4(r, 2);
-  a.z = shfl4(r, 3);
+// a = 2a as ec_dbl, the 8 products spread over the 4 lanes.  Four carry
+// passes instead of seven: X + Y, E and G stay unnormalized (limbs < 2^30),
+// which f_mul takes when the other factor is normalized (C, AB, H, F are),
+// and every value stays below 2^384 (E < 10q, G < 6q, F < 14q).
+__device__ __forceinline__ void ec_dbl4(Xyzt &a, int sel) {
+  F A, B, zz, S;
+  swap4(f_sqr(sel4(sel, a.x, a.y, a.z, f_add_nc(a.x, a.y))), A, B, zz, S);
+  const F C = f_add(zz, zz);                  // < 4q, normalized
+  const F AB = f_add(A, B);                   // < 4q, normalized
+  const F E = f_sub_nc_c<8>(S, AB);           // S + 8q - AB < 10q
+  const F G = f_sub_nc_c<4>(B, A);            // B - A + 4q < 6q
+  const F H = f_sub_kq_c<4>(f_zero(), AB);    // -(A + B) + 4q, normalized
+  const F Fm = f_sub_kq_c<8>(G, C);           // G - C + 8q < 14q, normalized
+  swap4(f_mul(sel4(sel, E, G, E, Fm), sel4(sel, Fm, H, H, G)), a.x, a.y, a.t, a.z);
 }
 
 // a += b as ec_add, the products of its first and last rounds spread over the lanes.
 __device__ __forceinline__ void ec_add4(Xyzt &a, const Xyzt &b, int sel, const FConsts &K) {
-  const F m1 = sel4(sel, f_sub_kq(a.y, a.x, K.q2), f_add(a.y, a.x), a.t, a.z);
-  const F m2 = sel4(sel, f_sub_kq(b.y, b.x, K.q2), f_add(b.y, b.x), b.t, b.z);
-  const F p = f_mul(m1, m2);
-  const F A = shfl4(p, 0), B = shfl4(p, 1), tt = shfl4(p, 2), zz = shfl4(p, 3);
+  const F m1 = sel4(sel, f_sub_kq_c<2>(a.y, a.x), f_add(a.y, a.x), a.t, a.z);
+  const F m2 = sel4(sel, f_sub_kq_c<2>(b.y, b.x), f_add(b.y, b.x), b.t, b.z);
+  F A, B, tt, zz;
+  swap4(f_mul(m1, m2), A, B, tt, zz);
   const F C = f_mul(tt, K.k2d);
   const F D = f_add(zz, zz);
-  const F E = f_sub_kq(B, A, K.q2);
+  const F E = f_sub_kq_c<2>(B, A);
   const F H = f_add(B, A);
-  const F Fm = f_sub_kq(D, C, K.q2);
+  const F Fm = f_sub_kq_c<2>(D, C);
   const F Gm = f_add(D, C);
-  const F r = f_mul(sel4(sel, E, Gm, E, Fm), sel4(sel, Fm, H, H, Gm));
-  a.x = shfl4(r, 0);
-  a.y = shfl4(r, 1);
-  a.t = shfl4(r, 2);
-  a.z = shfl4(r, 3);
+  swap4(f_mul(sel4(sel, E, Gm, E, Fm), sel4(sel, Fm, H, H, Gm)), a.x, a.y, a.t, a.z);
 }
 
 __device__ __forceinline__ void store_words(const F &v, uint64_t *out6) {
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(4) k_msm_finalize(const FinJobDev *jobs, const
   Xyzt acc = winsum[J.win0 + J.W - 1];
   for (int w = (int)J.W - 2; w >= 0; w--) {
     const uint32_t width = wins[J.win0 + w].width;
-    for (uint32_t i = 0; i < width; i++) ec_dbl4(acc, sel, K);
+    for (uint32_t i = 0; i < width; i++) ec_dbl4(acc, sel);
     ec_add4(acc, winsum[J.win0 + w], sel, K);
   }
   // Twisted Edwards -> Montgomery -> Weierstrass affine, in R' form: the

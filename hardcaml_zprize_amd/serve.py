@@ -55,7 +55,10 @@ class Job:
     def hbm_bytes(self):
         n = 1 << self.log_n
         if self.kind == NTT:
-            return 16 * n  # vector + 4-step scratch
+            # the vector, plus the 4-step scratch of the worker's plan for this
+            # size (kept for the worker's life, counted here per job so that
+            # the budget stays an upper bound)
+            return 16 * n
         # scalars; per (point, window) a digit, a pass-A entry and a sorted
         # entry (16 B); bucket sums, work-unit partials and window groups
         # (192 B points, generous); window c = min(20, max(8, log n - 6))

@@ -226,6 +226,7 @@ int zp_msm_batch_device(zp_msm_workspace *ws, const zp_msm_job *jobs, size_t cou
  * zp_mixed_batch_launch replays the whole graph on `stream` (asynchronous:
  * it recomputes every job from the current vectors and scalars);
  * zp_mixed_batch_wait synchronizes `stream` and writes each MSM job's out.
+ * The NTT vectors, scalar buffers and MSM contexts must outlive the batch.
  * No reference counterpart (the reference runs one transform / one MSM at a
  * time on its device, zprize/ntt/host/bench_throughput.cpp:61-107,
  * zprize/msm_pippenger/host/driver/driver.cpp:495-600). */
