@@ -640,7 +640,7 @@ def cpu_baseline_ntt(args, gpu_first_out, budget_s=12.0):
             "gpu_output_bit_exact": bool(same)}
 
 
-def config1(args, torch):
+def ntt_2p12_model(args, torch):
     """The 2^12 forward NTT as a 64 x 64 4-step on the CPU software model
     (the oracle's restated libs/hardcaml_ntt/src/reference_model.ml:92-105,
     1 thread), beside the direct CPU transform and the GPU's 2^12 transform
@@ -819,8 +819,8 @@ def main():
     if not args.skip_cpu:
         log("CPU baseline (oracle, 1 thread)")
         out["cpu_baseline"] = cpu_baseline_ntt(args, ntt["first_out"])
-        out["config1"] = config1(args, torch)
-        log(f"config 1: {out['config1']}")
+        out["ntt_2p12_model"] = ntt_2p12_model(args, torch)
+        log(f"2^12 four-step model: {out['ntt_2p12_model']}")
         if not args.skip_msm:
             out["msm"]["cpu_baseline"] = cpu_baseline_msm(args)
             log(f"MSM CPU baseline: {out['msm']['cpu_baseline']}")
