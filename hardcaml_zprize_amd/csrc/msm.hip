@@ -818,11 +818,10 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
       WinDev &wd = wins[w];
       uint32_t log_nb = 0;
       while ((1u << log_nb) < wd.nb) log_nb++;
-      // a shifted-table job's one window: about 2^16 entries per group,
-      // 2^11 to 2^14 groups (pass B's scattered writes stay within L2)
-      uint32_t hp = 11;
-      while (d.pre && hp < 14 && ((uint64_t)d.n >> 16) > (1ull << hp)) hp++;
-      const uint32_t hb = d.pre ? hp : 12, hd = d.pre ? hp : 11;
+      // 2^11 groups per window (a shifted-table job's one big window too):
+      // pass A's tiles of ~2^15 entries then write runs of ~16 entries per
+      // group, and pass B places a group of any size tile by tile.
+      const uint32_t hb = d.pre ? 11 : 12, hd = 11;
       const uint32_t lo = log_nb > hb ? log_nb - hb : 0;
       wd.lbits = std::max<uint32_t>(lo, lb_opt >= 0 ? (uint32_t)lb_opt : (log_nb > hd ? log_nb - hd : 0));
       wd.g0 = (uint32_t)GT;
@@ -877,7 +876,7 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
   ZP_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, ws.cnt, ws.off, 0u, (size_t)K + 1, rocprim::plus<uint32_t>(),
                                  stream));
   if (msd_path) {
-    ZP_TRY(wgrow(ws, ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1) * (any_pre ? 1 : 2)));
+    ZP_TRY(wgrow(ws, ws.tmp, ws.cap_tmp, std::max<uint64_t>(entries, 1)));
     ZP_TRY(wgrow(ws, ws.win_g0, ws.cap_wg0, std::max<size_t>(wins.size(), 1)));
   }
   if (sorted_path || msd_path) {
@@ -945,7 +944,7 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
 static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t stream, uint64_t *h_out) {
   const size_t count = P.jobs.size();
   const HostJob *jobs = P.jobs.data();
-  const bool sorted_path = P.sorted_path, msd_path = P.msd_path, any_pre = P.any_pre;
+  const bool sorted_path = P.sorted_path, msd_path = P.msd_path;
   const uint64_t K = P.K, points = P.points, counters = P.counters, n_max = P.n_max, GT = P.GT;
   const uint64_t max_units = P.max_units;
   const uint32_t max_nb = P.max_nb, max_lbits = P.max_lbits, max_groups = P.max_groups, max_chunks = P.max_chunks;
@@ -983,7 +982,12 @@ static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t st
     }
   } else if (msd_path) {
     const size_t lds_a = (size_t)max_groups * sizeof(uint32_t);
-    const size_t lds_b = ((2u << max_lbits) + 1024 + max_chunks + 1) * sizeof(uint32_t);
+    const size_t lds_sc = msd_scatter_lds_words(max_groups) * sizeof(uint32_t);
+    // pass B: cursors and tile ends per bucket, scan scratch, block tables,
+    // the chunk offsets, then as large a tile as the LDS holds
+    const size_t fixed_b = msd_group_fixed_words(max_lbits, max_chunks);
+    const uint32_t tile_b = (uint32_t)std::min<size_t>(MSD_B_PER * 1024, ((MSD_LDS_WORDS - fixed_b) / 1024) * 1024);
+    const size_t lds_b = (fixed_b + tile_b) * sizeof(uint32_t);
     if (points) {
       hipLaunchKernelGGL(k_digits_make, dim3(grid_for(points)), dim3(MSM_THREADS), 0, stream, B, ws.digits);
       ZP_HIP(hipGetLastError());
@@ -996,20 +1000,12 @@ static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t st
     ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb0, ws.counts, ws.scanned, 0u, (size_t)counters + 1,
                                    rocprim::plus<uint32_t>(), stream));
     if (points) {
-      if (any_pre)
-        hipLaunchKernelGGL(k_msd_scatter<true>, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
-                           ws.digits, ws.scanned, ws.tmp, ws.vals);
-      else
-        hipLaunchKernelGGL(k_msd_scatter<false>, dim3((unsigned)segs.size()), dim3(1024), lds_a, stream, B, ws.segs,
-                           ws.digits, ws.scanned, ws.tmp, ws.vals);
+      hipLaunchKernelGGL(k_msd_scatter, dim3((unsigned)segs.size()), dim3(1024), lds_sc, stream, B, ws.segs,
+                         ws.digits, ws.scanned, ws.tmp, ws.vals);
       ZP_HIP(hipGetLastError());
     }
-    if (any_pre)
-      hipLaunchKernelGGL(k_msd_group<true>, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt,
-                         (uint32_t)GT, max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
-    else
-      hipLaunchKernelGGL(k_msd_group<false>, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt,
-                         (uint32_t)GT, max_lbits, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
+    hipLaunchKernelGGL(k_msd_group, dim3((unsigned)GT), dim3(1024), lds_b, stream, B, ws.win_g0, Wt, (uint32_t)GT,
+                       max_lbits, max_chunks, tile_b, ws.scanned, ws.tmp, ws.vals, ws.off, ws.cnt);
     ZP_HIP(hipGetLastError());
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint64_t)K + 1, 0u);

@@ -30,6 +30,9 @@
 namespace zp {
 
 static constexpr int MSM_THREADS = 256;
+// Entries each thread of the sort kernels handles per loop round (their
+// global loads are issued together, before the dependent LDS work).
+static constexpr int SORT_ILP = 4;
 
 struct MsmJobDev {
   const PackedPre *pts;     // resident table of this job
@@ -242,17 +245,16 @@ __global__ void __launch_bounds__(1024) k_seg_scatter(BatchDev B, const SegDev *
   }
 }
 
-// ---- two-level sorted path (windows of any width up to 2^21 buckets) ----
+// ---- two-level sorted path (windows of any width up to 2^23 buckets) ----
 // Pass A partitions each window's entries by the high part of the bucket,
-// group h = b >> lbits (lbits per window), with per-(window, chunk) LDS
-// histograms as above; pass B sorts each group by the low lbits in a single
-// block and writes the
-// final bucket offsets.  Every write then lands in a run of consecutive
-// slots -- a chunk's entries of one group, a group's entries of one bucket --
-// where the single-level scatter writes about two entries per bucket per
-// chunk.  Group sizes are data dependent, so pass B finds its range in the
-// scanned pass-A counters on the device: group g's entries are
-// [scanned[i], scanned[i + C]) with i its first counter.
+// group h = b >> lbits (lbits per window, about 2^11 groups), with
+// per-(window, chunk) LDS histograms as above; pass B sorts each group by
+// the low lbits in a single block and writes the final bucket offsets.
+// Both passes stage tiles of entries in LDS and sort them there, so their
+// stores come out as runs of consecutive slots (a tile's entries of one
+// group, of one bucket).  Group sizes are data dependent, so pass B finds
+// its range in the scanned pass-A counters on the device: group g's entries
+// are [scanned[i], scanned[i + C]) with i its first counter.
 
 __host__ __device__ __forceinline__ uint32_t bucket_groups(uint32_t nb, uint32_t lbits) { return ((nb - 1) >> lbits) + 1; }
 
@@ -288,63 +290,195 @@ __global__ void __launch_bounds__(1024) k_msd_count(BatchDev B, const SegDev *se
   __syncthreads();
   const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
   const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const int32_t d = dig[i];
-    if (d != 0) lds_claim(hist, ((uint32_t)(d < 0 ? -d : d) - 1) >> lbits);
+  // SORT_ILP digits per thread and round: their loads are all in flight
+  // before the first LDS atomic.
+  for (uint32_t base = i0 + threadIdx.x; base < i1; base += SORT_ILP * blockDim.x) {
+    int32_t d[SORT_ILP];
+#pragma unroll
+    for (int k = 0; k < SORT_ILP; k++) {
+      const uint32_t i = base + k * blockDim.x;
+      d[k] = i < i1 ? dig[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < SORT_ILP; k++)
+      if (d[k] != 0) lds_claim(hist, ((uint32_t)(d[k] < 0 ? -d[k] : d[k]) - 1) >> lbits);
   }
   __syncthreads();
   uint32_t *dst = counts + jb.cnt0 + (uint64_t)(wd.g0 - jb.g0) * jb.chunks + sg.chunk;
   for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) dst[(uint64_t)h * jb.chunks] = hist[h];
 }
 
+// Block-wide exclusive scan of a[0, L) in LDS, in place, for a block of
+// 1024 threads (L <= 4096: each thread takes a run of <= 4 words); ws is 16
+// words of LDS.  Returns the total.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t L, uint32_t *ws) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x, nw = nt >> 6;
+  const uint32_t per = (L + nt - 1) / nt;
+  const uint32_t l0 = min(L, tid * per), l1 = min(L, l0 + per);
+  uint32_t sum = 0;
+  for (uint32_t l = l0; l < l1; l++) sum += a[l];
+  const uint32_t lane = tid & 63, wv = tid >> 6;
+  uint32_t x = sum;  // inclusive scan of the run sums within the wave
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wv] = x;
+  __syncthreads();
+  if (wv == 0) {
+    uint32_t t = lane < nw ? ws[lane] : 0u;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(t, o);
+      if (lane >= o) t += y;
+    }
+    if (lane < nw) ws[lane] = t;  // inclusive prefix of the wave totals
+  }
+  __syncthreads();
+  const uint32_t total = ws[nw - 1];
+  uint32_t run = x - sum + (wv ? ws[wv - 1] : 0u);
+  for (uint32_t l = l0; l < l1; l++) {
+    const uint32_t c = a[l];
+    a[l] = run;
+    run += c;
+  }
+  __syncthreads();
+  return total;
+}
+
+// Number of indices k in [0, L) with ends[k] <= q (ends ascending), i.e.
+// the bin holding slot q of a bin-sorted tile whose bin ends are ends[];
+// top = the largest power of two <= L.  Branch-free binary lifting.
+__device__ __forceinline__ uint32_t bin_of_slot(const uint32_t *ends, uint32_t L, uint32_t top, uint32_t q) {
+  uint32_t k = 0;
+  for (uint32_t st = top; st; st >>= 1)
+    if (k + st <= L && ends[k + st - 1] <= q) k += st;
+  return k;
+}
+
+// bin_of_slot restricted to bins [lo, hi] (the caller knows bin(q) lies
+// there): ceil(log2(hi - lo + 1)) lifting steps.
+__device__ __forceinline__ uint32_t bin_in_range(const uint32_t *ends, uint32_t lo, uint32_t hi, uint32_t q) {
+  uint32_t st = 1;
+  while (st * 2 <= hi - lo) st <<= 1;
+  uint32_t k = lo;
+  for (; st; st >>= 1)
+    if (k + st <= hi && ends[k + st - 1] <= q) k += st;
+  return k;
+}
+
+// For the slots base .. base + count - 1 (count > 0) in 64-slot blocks:
+// first[b] = the bin of slot base + 64 b, and first[nblk] = the bin of the
+// last slot, so that slot base + i lies in a bin of
+// [first[i >> 6], first[(i >> 6) + 1]] and the 64 lanes of a wave
+// (consecutive slots) read the same two words.
+__device__ __forceinline__ void block_first_bins(const uint32_t *ends, uint32_t L, uint32_t top, uint32_t base,
+                                                 uint32_t count, uint32_t *first) {
+  const uint32_t nblk = (count + 63) >> 6;
+  for (uint32_t b = threadIdx.x; b <= nblk; b += blockDim.x)
+    first[b] = bin_of_slot(ends, L, top, base + (b < nblk ? b << 6 : count - 1));
+}
+
 // Pass A scatter, grouped by (window, group), chunks in order within a
-// group.  COMPACT (batches with a shifted-table job): 4-byte entries
-// tmp[pos] = (i - chunk start) << (lbits + 1) | low lbits of the bucket << 1
-// | sign (pass B recovers the chunk from the group's chunk offsets; the host
-// keeps chunk_len << (lbits + 1) within 32 bits).  Otherwise 8-byte entries
-// (point | sign << 31, bucket), which the per-window buckets' scatter of 2^11
-// runs per chunk writes faster.  A window whose groups are single buckets
-// (lbits = 0) is sorted by now: its entries go straight to vals (same slots,
-// point | sign << 31), and pass B only writes their offsets.
-template <bool COMPACT>
+// group, as 4-byte entries tmp[pos] = (i - chunk start) << (lbits + 1) |
+// low lbits of the bucket << 1 | sign (pass B recovers the chunk from the
+// group's chunk offsets; the host keeps chunk_len << (lbits + 1) within 32
+// bits).  A window whose groups are single buckets (lbits = 0) is sorted by
+// now: its entries go straight to vals (same slots, point | sign << 31), and
+// pass B only writes their offsets.
+//
+// The chunk is processed in tiles of MSD_A_TILE entries.  Each tile is
+// sorted by group in LDS (count, scan, place) and then written out slot by
+// slot, so that consecutive lanes store a group's run of the tile to
+// consecutive addresses: a few wide writes per run, where one 4-byte store
+// per entry into 2^11 places at once left the L2 nothing to merge (every
+// entry became its own 32-byte memory write).  The group's cursor in the
+// chunk's run advances by the tile's count.
+static constexpr int MSD_A_PER = 30;                           // entries per thread
+static constexpr uint32_t MSD_A_TILE = (uint32_t)MSD_A_PER * 1024;  // 120 KiB of LDS
+// LDS words of k_msd_scatter for at most H <= 4096 groups per window.
+__host__ __device__ constexpr size_t msd_scatter_lds_words(uint32_t H) {
+  return MSD_A_TILE + 2 * (size_t)H + 16 + MSD_A_TILE / 64 + 1;
+}
+
 __global__ void __launch_bounds__(1024) k_msd_scatter(BatchDev B, const SegDev *segs, const int32_t *digits,
                                                       const uint32_t *scanned, uint32_t *tmp, uint32_t *vals) {
-  extern __shared__ uint32_t cursor[];
+  extern __shared__ uint32_t lds[];
+  constexpr uint32_t NONE = 0xffffffffu;
   const SegDev sg = segs[blockIdx.x];
   const MsmJobDev jb = B.jobs[sg.job];
   const WinDev wd = B.wins[sg.win];
   const uint32_t lbits = wd.lbits;
   const uint32_t H = bucket_groups(wd.nb, lbits);
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  uint32_t *ent = lds;                     // the tile sorted by group
+  uint32_t *cursor = lds + MSD_A_TILE;     // per group: next slot of the chunk's run
+  uint32_t *lend = cursor + H;             // per group: tile count -> start -> end
+  uint32_t *ws = lend + H;                 // scan scratch
+  uint32_t *first = ws + 16;               // first group of each 64-slot block
   const uint32_t *src = scanned + jb.cnt0 + (uint64_t)(wd.g0 - jb.g0) * jb.chunks + sg.chunk;
-  for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) cursor[h] = src[(uint64_t)h * jb.chunks];
-  __syncthreads();
+  for (uint32_t h = tid; h < H; h += nt) cursor[h] = src[(uint64_t)h * jb.chunks];
+  uint32_t top = 1;
+  while (top * 2 <= H) top <<= 1;
   const uint32_t i0 = sg.chunk * jb.chunk_len, i1 = min(jb.n, i0 + jb.chunk_len);
   const int32_t *dig = digits + jb.dig0 + (uint64_t)(sg.win - jb.win0) * jb.n;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const int32_t d = dig[i];
-    if (d == 0) continue;
-    const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
-    const uint32_t pos = lds_claim(cursor, b >> lbits);
-    const uint32_t sign = d < 0 ? 1u : 0u;
-    if (lbits == 0)
-      vals[pos] = i | sign << 31;
-    else if (COMPACT)
-      tmp[pos] = (i - i0) << (lbits + 1) | (b & ((1u << lbits) - 1)) << 1 | sign;
-    else
-      reinterpret_cast<uint2 *>(tmp)[pos] = make_uint2(i | sign << 31, b);
+  for (uint32_t t0 = i0; t0 < i1; t0 += MSD_A_TILE) {
+    for (uint32_t h = tid; h < H; h += nt) lend[h] = 0;
+    __syncthreads();
+    uint32_t key[MSD_A_PER], e[MSD_A_PER];
+#pragma unroll
+    for (int k = 0; k < MSD_A_PER; k++) {
+      const uint32_t i = t0 + tid + k * nt;
+      const int32_t d = i < i1 ? dig[i] : 0;
+      key[k] = NONE;
+      e[k] = 0;
+      if (d != 0) {
+        const uint32_t b = (uint32_t)(d < 0 ? -d : d) - 1;
+        const uint32_t sign = d < 0 ? 1u : 0u;
+        key[k] = b >> lbits;
+        e[k] = lbits == 0 ? i | sign << 31 : (i - i0) << (lbits + 1) | (b & ((1u << lbits) - 1)) << 1 | sign;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < MSD_A_PER; k++)
+      if (key[k] != NONE) lds_claim(lend, key[k]);
+    __syncthreads();
+    const uint32_t tot = block_excl_scan(lend, H, ws);  // lend: each group's first slot in the tile
+#pragma unroll
+    for (int k = 0; k < MSD_A_PER; k++)
+      if (key[k] != NONE) ent[lds_claim(lend, key[k])] = e[k];  // lend ends as each group's end
+    __syncthreads();
+    if (tot) block_first_bins(lend, H, top, 0, tot, first);
+    __syncthreads();
+    uint32_t *dst = lbits == 0 ? vals : tmp;
+    for (uint32_t q = tid; q < tot; q += nt) {
+      const uint32_t h = bin_in_range(lend, first[q >> 6], first[(q >> 6) + 1], q);
+      dst[cursor[h] + q - (h ? lend[h - 1] : 0u)] = ent[q];
+    }
+    __syncthreads();
+    for (uint32_t h = tid; h < H; h += nt) cursor[h] += lend[h] - (h ? lend[h - 1] : 0u);
+    __syncthreads();
   }
 }
 
 // Pass B: block g sorts bucket group g (window w, high part h) by the low
-// lbits of its window: histogram, exclusive scan, off/cnt of the group's
-// buckets, then placement into vals (point | sign << 31, the point being its
-// chunk's first point plus the offset pass A stored).  The last group also
-// writes off[K] (the total).  LDS: 2 * 2^max_lbits + 1024 + max_chunks + 1 words.
-template <bool COMPACT>
+// lbits of its window.  A histogram over the whole group gives the bucket
+// offsets and counts; then the group is placed tile by tile (tile_n entries,
+// at most MSD_B_PER per thread): each tile is sorted by bucket in LDS and
+// written out slot by slot, so a bucket's run of the tile is stored by
+// consecutive lanes (vals = point | sign << 31, the point being its chunk's
+// first point plus the offset pass A stored).  The last group also writes
+// off[K] (the total).  LDS words: msd_group_fixed_words + tile_n.
+static constexpr int MSD_B_PER = 32;
+static constexpr size_t MSD_LDS_WORDS = 160 * 1024 / 4;  // LDS of one gfx950 workgroup
+__host__ __device__ constexpr size_t msd_group_fixed_words(uint32_t max_lbits, uint32_t max_chunks) {
+  return (2u << max_lbits) + 16 + 2 * (MSD_B_PER * 1024 / 64 + 1) + max_chunks + 1;
+}
 __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *win_g0, uint32_t Wt, uint32_t GT,
-                                                    uint32_t max_lbits, const uint32_t *scanned, const uint32_t *tmp,
-                                                    uint32_t *vals, uint32_t *off, uint32_t *cnt) {
+                                                    uint32_t max_lbits, uint32_t max_chunks, uint32_t tile_n,
+                                                    const uint32_t *scanned, const uint32_t *tmp, uint32_t *vals,
+                                                    uint32_t *off, uint32_t *cnt) {
   extern __shared__ uint32_t lds[];
+  constexpr uint32_t NONE = 0xffffffffu;
   const uint32_t g = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   const uint32_t w = upper_index(win_g0, Wt, g);
   const WinDev wd = B.wins[w];
@@ -353,8 +487,6 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
   const uint32_t start = scanned[ci], end = scanned[ci + jb.chunks];
   const uint32_t b0 = (g - wd.g0) << wd.lbits;
   const uint32_t nl = min(1u << wd.lbits, wd.nb - b0);  // buckets in this group
-  uint32_t *hist = lds, *cur = lds + (1u << max_lbits), *part = cur + (1u << max_lbits);
-  uint32_t *cs = part + nt;  // the group's chunk offsets: chunk c's entries are [cs[c], cs[c + 1])
   if (wd.lbits == 0) {  // one bucket, placed by pass A
     if (tid == 0) {
       off[wd.base + b0] = start;
@@ -363,54 +495,80 @@ __global__ void __launch_bounds__(1024) k_msd_group(BatchDev B, const uint32_t *
     }
     return;
   }
+  uint32_t *cur = lds, *lend = cur + (1u << max_lbits), *ws = lend + (1u << max_lbits);
+  uint32_t *cfirst = ws + 16;                               // chunk of each 64-position block of the tile
+  uint32_t *bfirst = cfirst + (MSD_B_PER * 1024 / 64 + 1);  // bucket of each 64-slot block
+  uint32_t *cs = bfirst + (MSD_B_PER * 1024 / 64 + 1);      // chunk c's entries are [cs[c], cs[c + 1])
+  uint32_t *ent = cs + max_chunks + 1;
   const uint32_t lmask = (1u << wd.lbits) - 1;
-  for (uint32_t l = tid; l < nl; l += nt) hist[l] = 0;
+  for (uint32_t l = tid; l < nl; l += nt) lend[l] = 0;
   for (uint32_t c = tid; c <= jb.chunks; c += nt) cs[c] = scanned[ci + c];
   __syncthreads();
-  const uint2 *tmp2 = reinterpret_cast<const uint2 *>(tmp);
-  for (uint32_t p = start + tid; p < end; p += nt)
-    lds_claim(hist, COMPACT ? (tmp[p] >> 1) & lmask : tmp2[p].y & lmask);
-  __syncthreads();
-  // exclusive scan of hist[0, nl): each thread sums a run of `per`, a
-  // Hillis-Steele scan over the 1024 run sums, then each run is expanded.
-  const uint32_t per = (nl + nt - 1) / nt;
-  const uint32_t l0 = min(nl, tid * per), l1 = min(nl, l0 + per);
-  uint32_t sum = 0;
-  for (uint32_t l = l0; l < l1; l++) sum += hist[l];
-  part[tid] = sum;
-  __syncthreads();
-  for (uint32_t st = 1; st < nt; st <<= 1) {
-    const uint32_t v = part[tid] + (tid >= st ? part[tid - st] : 0u);
-    __syncthreads();
-    part[tid] = v;
-    __syncthreads();
+  for (uint32_t base = start + tid; base < end; base += SORT_ILP * nt) {
+    uint32_t key[SORT_ILP];
+#pragma unroll
+    for (int k = 0; k < SORT_ILP; k++) {
+      const uint32_t p = base + k * nt;
+      key[k] = p < end ? (tmp[p] >> 1) & lmask : NONE;
+    }
+#pragma unroll
+    for (int k = 0; k < SORT_ILP; k++)
+      if (key[k] != NONE) lds_claim(lend, key[k]);
   }
-  uint32_t run = start + part[tid] - sum;  // exclusive prefix of this thread's run
-  for (uint32_t l = l0; l < l1; l++) {
-    const uint32_t c = hist[l];
-    off[wd.base + b0 + l] = run;
-    cnt[wd.base + b0 + l] = c;
-    cur[l] = run;
-    run += c;
+  __syncthreads();
+  for (uint32_t l = tid; l < nl; l += nt) cnt[wd.base + b0 + l] = lend[l];
+  block_excl_scan(lend, nl, ws);
+  for (uint32_t l = tid; l < nl; l += nt) {
+    off[wd.base + b0 + l] = start + lend[l];
+    cur[l] = start + lend[l];
   }
   if (g == GT - 1 && tid == 0) off[B.K] = end;
-  __syncthreads();
-  for (uint32_t p = start + tid; p < end && !COMPACT; p += nt) {
-    const uint2 e = tmp2[p];
-    vals[lds_claim(cur, e.y & lmask)] = e.x;
-  }
-  for (uint32_t p = start + tid; p < end && COMPACT; p += nt) {
-    const uint32_t e = tmp[p];
-    uint32_t lo = 0, hi = jb.chunks;  // cs[lo] <= p < cs[hi]: find p's chunk
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (cs[mid] <= p)
-        lo = mid;
-      else
-        hi = mid;
+  uint32_t top = 1;
+  while (top * 2 <= nl) top <<= 1;
+  uint32_t ctop = 1;  // the largest power of two <= C - 1
+  while (ctop * 2 < jb.chunks) ctop <<= 1;
+  const uint32_t per = tile_n / nt;
+  for (uint32_t t0 = start; t0 < end; t0 += tile_n) {
+    const uint32_t t1 = min(end, t0 + tile_n);
+    __syncthreads();  // cur (first tile) / the previous tile's reads of lend, bfirst and ent
+    for (uint32_t l = tid; l < nl; l += nt) lend[l] = 0;
+    // p's chunk = the number of chunk starts cs[1 .. C - 1] at or before p (cs[0] = start <= p)
+    if (jb.chunks > 1) block_first_bins(cs + 1, jb.chunks - 1, ctop, t0, t1 - t0, cfirst);
+    __syncthreads();
+    uint32_t key[MSD_B_PER], val[MSD_B_PER];
+#pragma unroll
+    for (int k = 0; k < MSD_B_PER; k++) {
+      const uint32_t p = t0 + tid + k * nt;
+      key[k] = NONE;
+      val[k] = 0;
+      if (k < (int)per && p < t1) {
+        const uint32_t e = tmp[p];
+        key[k] = (e >> 1) & lmask;
+        uint32_t c = 0;
+        if (jb.chunks > 1) {
+          const uint32_t bk = (p - t0) >> 6;
+          c = bin_in_range(cs + 1, cfirst[bk], cfirst[bk + 1], p);
+        }
+        val[k] = (c * jb.chunk_len + (e >> (wd.lbits + 1))) | (e & 1u) << 31;
+      }
     }
-    const uint32_t i = lo * jb.chunk_len + (e >> (wd.lbits + 1));
-    vals[lds_claim(cur, (e >> 1) & lmask)] = i | (e & 1u) << 31;
+#pragma unroll
+    for (int k = 0; k < MSD_B_PER; k++)
+      if (key[k] != NONE) lds_claim(lend, key[k]);
+    __syncthreads();
+    const uint32_t tot = block_excl_scan(lend, nl, ws);
+#pragma unroll
+    for (int k = 0; k < MSD_B_PER; k++)
+      if (key[k] != NONE) ent[lds_claim(lend, key[k])] = val[k];
+    __syncthreads();
+    if (tot) block_first_bins(lend, nl, top, 0, tot, bfirst);
+    __syncthreads();
+    for (uint32_t q = tid; q < tot; q += nt) {
+      const uint32_t l = bin_in_range(lend, bfirst[q >> 6], bfirst[(q >> 6) + 1], q);
+      vals[cur[l] + q - (l ? lend[l - 1] : 0u)] = ent[q];
+    }
+    __syncthreads();
+    for (uint32_t l = tid; l < nl; l += nt) cur[l] += lend[l] - (l ? lend[l - 1] : 0u);
   }
 }
 

@@ -46,8 +46,10 @@ __device__ __forceinline__ uint64_t gl_shl(uint64_t a) {
 //                after a carry the result is already < p.
 //   32 < E < 64  v 2^(E-32) = lo + hi 2^64, so v 2^E = lo 2^32 - hi
 //                = (lo_lo + lo_hi) 2^32 - (lo_hi + hi) = A 2^32 - B
-//                = A0 2^32 + A1 eps - B  (A, B < 2^33), in (-2^33, p): add p if
-//                negative.
+//                = (A0 + A1 - B1) 2^32 - (A1 + B0)  (A = A1 2^32 + A0,
+//                B = B1 2^32 + B0, A1 and B1 carries), in (-2^33, p): add p
+//                if negative.  The 32-bit words go through one carry chain
+//                (no 64-bit zero-extended temporaries, no compares).
 //   E >= 64      v 2^E = -v 2^-K (K = 96 - E in (0, 32]) = M eps - q with
 //                q = v >> K and M = low word of v << (32 - K); M eps < p.
 template <int E>
@@ -65,11 +67,17 @@ __host__ __device__ __forceinline__ uint64_t gl_shl_lz(uint64_t v) {
     const uint32_t lo_lo = vl << F;
     const uint32_t lo_hi = (uint32_t)(v >> (32 - F));
     const uint32_t hi = vh >> (32 - F);
-    const uint64_t A = (uint64_t)lo_lo + lo_hi;
-    const uint64_t B = (uint64_t)lo_hi + hi;
-    const uint64_t Y = (uint64_t)(uint32_t)(A >> 32) * GL_EPS + (A << 32);
-    const uint64_t d = Y - B;
-    return Y < B ? d + GL_P : d;
+    unsigned A1, B1, b1, b2, c2;
+    const uint32_t A0 = __builtin_addc(lo_lo, lo_hi, 0u, &A1);
+    const uint32_t B0 = __builtin_addc(lo_hi, hi, 0u, &B1);
+    // low word -(A1 + B0) with its borrow b1; high word A0 + A1 - B1 - b1.
+    // A0 + A1 never carries: A1 = 1 means lo_lo + lo_hi >= 2^32, so A0 <=
+    // 2^32 - 2.  The high word's borrow b2 is then the sign of the value.
+    const uint32_t r_lo = __builtin_subc(0u, B0, A1, &b1);
+    const uint32_t t_hi = __builtin_addc(A0, 0u, A1, &c2);
+    const uint32_t r_hi = __builtin_subc(t_hi, B1, b1, &b2);
+    const uint64_t r = ((uint64_t)r_hi << 32) | r_lo;
+    return b2 ? r + GL_P : r;  // + p = - eps (mod 2^64)
   } else {
     constexpr int K = 96 - E;
     const uint32_t M = K == 32 ? vl : (vl << ((32 - K) & 31));
