@@ -23,6 +23,7 @@
 #include "common.hpp"
 #include "gl64.hpp"
 #include "ntt4096.hpp"
+#include "ntt4096r16.hpp"
 #include "ntt_kernels.hpp"
 
 namespace zp {
@@ -46,6 +47,7 @@ struct NttTables {
   uint64_t *d_lo[2] = {nullptr, nullptr};   // w_n^i, i < 4096
   uint64_t *d_hi[2] = {nullptr, nullptr};   // w_n^(4096 i), i < max(1, n / 4096)
   uint64_t *d_4096[2] = {nullptr, nullptr}; // [k][a] = w_4096^(a k), a, k < 64
+  uint64_t *d_full[2] = {nullptr, nullptr}; // w_4096^i, i < 4096
   uint64_t n_inv = 1;
 };
 
@@ -89,6 +91,8 @@ static int make_tables(unsigned log_n, NttTables **out) {
     for (int k = 0; k < 64; k++)
       for (int a = 0; a < 64; a++) ka[k * 64 + a] = full[a * k];
     ZP_HIP(hipMemcpy(t->d_4096[dir], ka.data(), 4096 * 8, hipMemcpyHostToDevice));
+    ZP_HIP(hipMalloc(&t->d_full[dir], 4096 * 8));
+    ZP_HIP(hipMemcpy(t->d_full[dir], full.data(), 4096 * 8, hipMemcpyHostToDevice));
     ZP_HIP(hipMalloc(&t->d_lo[dir], 4096 * 8));
     ZP_HIP(hipMalloc(&t->d_hi[dir], n_hi * 8));
     ZP_HIP(hipMemcpy(t->d_sub[dir], sub.data(), 2048 * 8, hipMemcpyHostToDevice));
@@ -193,6 +197,27 @@ static int launch_4096_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out,
   return ZP_OK;
 }
 
+// Pass 2 of the 2^24 transform: 16 x 16 x 16 row DFTs (ntt4096r16.hpp).
+static int launch_r16_rows(const NttTables *t, uint64_t *d_in, uint64_t *d_out, int dir, hipStream_t stream,
+                           const JobTabs &jt) {
+  Ntt16Args a{};
+  a.n1 = 4096;
+  a.n2 = 4096;
+  a.tw_full = t->d_full[dir];
+  a.tiles = 4096 / R16_T;
+  a.in = d_in;
+  a.out = d_out;
+  a.in_tab = jt.scratch;
+  a.out_tab = jt.data;
+  const dim3 grid(a.tiles, jt.jobs);
+  if (dir)
+    hipLaunchKernelGGL(k_ntt4096r16_rows<1>, grid, dim3(256 * R16_T), r16_lds_bytes(), stream, a);
+  else
+    hipLaunchKernelGGL(k_ntt4096r16_rows<0>, grid, dim3(256 * R16_T), r16_lds_bytes(), stream, a);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
 // ----------------------------------------------------------- REVERSE core --
 // The FPGA NTT build's memory-path debug core (zprize/ntt/host/driver.h:18,
 // reference.cpp:60-84): the two phases of the 4-step transform with each row
@@ -250,9 +275,13 @@ static int launch_reverse_phase(const NttTables *t, uint64_t *d_data, uint64_t *
   return ZP_OK;
 }
 
+static bool g_tile_rows_env = false;
 static void tile_widths() {
   if (g_tile_cols < 0) g_tile_cols = env_tile("ZP_NTT_TILE_COLS", 8);
-  if (g_tile_rows < 0) g_tile_rows = env_tile("ZP_NTT_TILE_ROWS", 8);
+  if (g_tile_rows < 0) {
+    g_tile_rows = env_tile("ZP_NTT_TILE_ROWS", 8);
+    g_tile_rows_env = getenv("ZP_NTT_TILE_ROWS") || getenv("ZP_NTT_TILE");
+  }
 }
 
 // One pass of an in-place transform (pass 1 = column DFTs + 4-step twiddle,
@@ -267,8 +296,10 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
     if (pass == 1)
       return g_tile_cols == 4 ? launch_4096_cols<4>(t, d_data, d_scratch, dir, stream, jt)
                               : launch_4096_cols<8>(t, d_data, d_scratch, dir, stream, jt);
-    return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
-                            : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
+    // ZP_NTT_TILE_ROWS=4 or 8 selects the 64 x 64 row kernel instead
+    if (g_tile_rows_env) return g_tile_rows == 4 ? launch_4096_rows<4>(t, d_scratch, d_data, dir, stream, jt)
+                                                 : launch_4096_rows<8>(t, d_scratch, d_data, dir, stream, jt);
+    return launch_r16_rows(t, d_scratch, d_data, dir, stream, jt);
   }
   if (pass == 1 && sh.log_n1 == 12 && sh.log_n2 >= 3) {
     // 4096-point columns in registers; pass 2 (rows shorter than 4096) stays
