@@ -743,7 +743,7 @@ def ntt_roofline(args, ntt):
     achieved = alg / (ntt["ms_per_step"] * 1e-3) / 1e9
     per_kernel = []
     traffic = 0
-    for name, kms in (("k_ntt4096_cols", ntt["ms_pass1"]), ("k_ntt4096_rows", ntt["ms_pass2"])):
+    for name, kms in (("k_ntt4096_cols", ntt["ms_pass1"]), ("k_ntt4096r16_rows", ntt["ms_pass2"])):
         a = 8 * n / (kms * 1e-3) / 1e9
         t = measured_traffic(name) if args.log_n == 24 else None
         traffic = None if (t is None or traffic is None) else traffic + t
