@@ -1026,8 +1026,8 @@ static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t st
     }
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(MSM_THREADS), 0, stream, ws.umax, (uint64_t)1, 0u);
-  hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.cnt, (uint32_t)K,
-                     log_u, ws.nunits, ws.umax);
+  hipLaunchKernelGGL(k_units_per_bucket, dim3(grid_for(K + 1)), dim3(MSM_THREADS), 0, stream, ws.off, ws.cnt,
+                     (uint32_t)K, log_u, ws.nunits, ws.umax);
   ZP_HIP(hipGetLastError());
   size_t tb = ws.cap_scan;
   ZP_HIP(rocprim::exclusive_scan(ws.scan_tmp, tb, ws.nunits, ws.ustart, 0u, (size_t)K + 1,
@@ -1036,12 +1036,14 @@ static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t st
   hipLaunchKernelGGL(k_fill_units, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.ustart, (uint32_t)K,
                      total_units, ws.ubucket);
   ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(max_units, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream, B,
-                     ws.vals, ws.off, ws.cnt, ws.ustart, ws.ubucket, total_units, log_u, ws.partial, KC);
+  const uint64_t slices = (std::max<uint64_t>(P.entries, 1) + (1ull << log_u) - 1) >> log_u;
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(slices, MSM_THREADS, 1u << 20)), dim3(MSM_THREADS), 0, stream, B,
+                     ws.vals, ws.off, ws.ustart, (uint32_t)K, log_u, ws.partial, KC);
   ZP_HIP(hipGetLastError());
   // A bucket holds at most n_max entries (one digit per point and window),
-  // so at most ceil(n_max / U) units: that many tree levels suffice.
-  const uint64_t units_per_bucket = (n_max + (1ull << log_u) - 1) >> log_u;
+  // so it touches at most ceil(n_max / U) + 1 slices: that many tree levels
+  // suffice.
+  const uint64_t units_per_bucket = ((n_max + (1ull << log_u) - 1) >> log_u) + 1;
   for (uint64_t stride = 1; stride < units_per_bucket; stride <<= 1) {
     hipLaunchKernelGGL(k_unit_tree, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ubucket,
                        ws.ustart, ws.nunits, total_units, ws.umax, (uint32_t)stride, KC);

@@ -9,18 +9,20 @@
 //                    (libs/pippenger/src/controller.ml:128-132);
 //   scan             bucket offsets and work-unit offsets;
 //   k_digits_scatter counting-sort placement: vals[pos] = point | sign << 31;
-//   k_fill_units     unit -> bucket map (a unit is <= U entries of one bucket);
-//   k_accumulate     one thread per unit: mixed twisted-Edwards additions of
-//                    the unit's points into an extended accumulator
-//                    (the FPGA bucket pipeline, top.ml / mixed_add_precompute.ml);
+//   k_fill_units     unit -> bucket map (a unit is the part of one bucket
+//                    inside one slice of U sorted entries);
+//   k_accumulate     one thread per slice: mixed twisted-Edwards additions
+//                    of the slice's points into an extended accumulator per
+//                    unit (the FPGA bucket pipeline, top.ml /
+//                    mixed_add_precompute.ml);
 //   k_unit_tree      per bucket: log-depth pairwise sum of its unit partials;
 //   k_bucket_sum     per bucket: the reduced partial (identity if empty);
 //   k_window_groups  per group of g buckets: running-sum reduction
 //                    (driver.cpp:318-339, pippenger.h:25-33) done as
 //                    sum_b (b - lo + 1) S_b + lo * sum_b S_b;
 //   k_window_sum     one workgroup per window: tree sum of the group results.
-// The window combine and the affine conversion run on the host (a few
-// thousand field operations per job, the part of driver.cpp:340-375 left).
+// The window combine and the affine conversion follow in k_msm_finalize
+// (msm_finalize.hpp; driver.cpp:340-375 does them on the FPGA's host).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -589,16 +591,20 @@ __global__ void k_bucket_counts_from_off(const uint32_t *off, uint32_t K, uint32
     cnt[k] = off[k + 1] - off[k];
 }
 
-// nunits[k] = ceil(cnt[k] / U); k == K gets 0 (so the scan's last entry is the total).
-// *umax (zeroed by the caller) becomes the largest unit count of any bucket
-// when that exceeds 1 (a wave maximum, then one atomic per wave).
-__global__ void k_units_per_bucket(const uint32_t *cnt, uint32_t K, uint32_t log_u, uint32_t *nunits,
-                                   uint32_t *umax) {
+// Work units: the sorted entries are cut into slices of U = 2^log_u (one
+// thread each, k_accumulate), and a unit is the part of one bucket inside
+// one slice.  nunits[k] = the number of slices bucket k's entries
+// [off[k], off[k] + cnt[k]) touch (0 when empty); k == K gets 0 (so the
+// scan's last entry is the total).  *umax (zeroed by the caller) becomes the
+// largest unit count of any bucket when that exceeds 1 (a wave maximum,
+// then one atomic per wave).
+__global__ void k_units_per_bucket(const uint32_t *off, const uint32_t *cnt, uint32_t K, uint32_t log_u,
+                                   uint32_t *nunits, uint32_t *umax) {
   for (uint32_t base = blockIdx.x * blockDim.x; base <= K; base += gridDim.x * blockDim.x) {  // block-uniform
     const uint32_t k = base + threadIdx.x;
     uint32_t v = 0;
     if (k <= K) {
-      v = k < K ? (cnt[k] + (1u << log_u) - 1) >> log_u : 0;
+      v = k < K && cnt[k] ? ((off[k] + cnt[k] - 1) >> log_u) - (off[k] >> log_u) + 1 : 0;
       nunits[k] = v;
     }
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
@@ -626,22 +632,39 @@ __global__ void k_fill_units(const uint32_t *ustart, uint32_t K, const uint32_t 
   }
 }
 
+// One thread per slice of U = 2^log_u consecutive sorted entries (the last
+// one shorter), so every lane of a wave makes the same number of mixed
+// additions whatever the bucket sizes (with one thread per bucket, lanes
+// idle while the wave's largest bucket finishes).  A slice's entries belong
+// to one or more consecutive buckets; each (bucket, slice) piece is a work
+// unit whose partial goes to partial[u], u = ustart[k] + the slice's rank
+// among the slices bucket k touches, and k_unit_tree / k_bucket_sum add a
+// bucket's pieces.  The bucket changes inside the one flat loop (a short
+// divergent branch), so lanes never wait for each other's piece lengths.
 __global__ void __launch_bounds__(MSM_THREADS) k_accumulate(BatchDev B, const uint32_t *vals, const uint32_t *off,
-                                                            const uint32_t *cnt, const uint32_t *ustart,
-                                                            const uint32_t *ubucket, const uint32_t *total_units,
-                                                            uint32_t log_u, Xyzt *partial, FConsts K) {
-  const uint32_t total = *total_units;
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < total; u += gridDim.x * blockDim.x) {
-    const uint32_t k = ubucket[u];
+                                                            const uint32_t *ustart, uint32_t K, uint32_t log_u,
+                                                            Xyzt *partial, FConsts KC) {
+  const uint32_t total = off[K];
+  const uint32_t slices = (total + (1u << log_u) - 1) >> log_u;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < slices; t += gridDim.x * blockDim.x) {
+    const uint32_t e0 = t << log_u, e1 = min(total, e0 + (1u << log_u));
+    uint32_t k = upper_index(off, K, e0);  // the non-empty bucket holding e0 (empty ones share its offset)
+    uint32_t u = ustart[k] + (t - (off[k] >> log_u));
+    uint32_t seg_end = min(off[k + 1], e1);
     const PackedPre *pts = B.jobs[upper_index(B.job_b0, B.J, k)].pts;
-    const uint32_t s = u - ustart[k];
-    const uint32_t b0 = off[k] + (s << log_u);
-    const uint32_t b1 = min(off[k] + cnt[k], b0 + (1u << log_u));
-    Xyzt acc = ec_identity(K);
-    for (uint32_t e = b0; e < b1; e++) {
+    Xyzt acc = ec_identity(KC);
+    for (uint32_t e = e0; e < e1; e++) {
+      if (e == seg_end) {  // bucket k ends inside the slice: store its piece, go to the next bucket
+        partial[u] = acc;
+        acc = ec_identity(KC);
+        while (off[k + 1] <= e) k++;  // off[K] = total > e ends the walk
+        u = ustart[k];
+        seg_end = min(off[k + 1], e1);
+        pts = B.jobs[upper_index(B.job_b0, B.J, k)].pts;
+      }
       const uint32_t v = vals[e];
       const PrePt p = pre_unpack(pts[v & 0x7fffffffu]);
-      ec_madd(acc, p, (v >> 31) != 0, K);
+      ec_madd(acc, p, (v >> 31) != 0, KC);
     }
     partial[u] = acc;
   }
