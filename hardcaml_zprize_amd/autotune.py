@@ -3,7 +3,8 @@
 For each size class log2 N it times `window` candidates around the library's
 default policy on the GPU and keeps the fastest; the result is a small JSON
 table that `MsmWorkspace.run` jobs (per-job `window`) and the serving loop
-consult.  `ZP_NTT_TILE` (4 or 8) selects the 4096-point NTT kernels' tile.
+consult.  `ZP_NTT_TILE` (4 or 8) selects the 64 x 64 NTT kernels' tile (set, it
+also puts the 2^24 transform's second pass on the 64 x 64 row kernel).
 
     python -m hardcaml_zprize_amd.autotune --min-log 10 --max-log 22 --options precompute=auto
 """

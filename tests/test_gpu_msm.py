@@ -396,6 +396,27 @@ def test_two_level_sort_hot_buckets():
             assert dt < 5.0, (name, dt)
 
 
+@pytest.mark.parametrize("options", ["window=16,sort=msd", "window=16,sort=msd,log_u=4", "window=13,sort=atomic,log_u=8"])
+def test_slices_across_many_buckets(options):
+    """Scalars 1..n (window 0 holds about one entry per bucket, the higher
+    windows are empty) and n copies of a few values (a handful of full
+    buckets between long runs of empty ones): an accumulation slice then
+    crosses hundreds of bucket ends, or lies inside one bucket; closed form
+    ((sum s_i k_i) mod r) G both times."""
+    rng = random.Random(810)
+    n = 1 << 16
+    ks = [rng.randrange(1, R) for _ in range(n)]
+    pts = zmsm.generate_points(_words(ks, 4))
+    few = [rng.randrange(R) for _ in range(5)]
+    cases = {"consecutive": [i + 1 for i in range(n)], "five_values": [few[i % 5] for i in range(n)]}
+    with zmsm.MsmContext(pts, options) as ctx:
+        for name, s in cases.items():
+            _say(f"msm slices {options} {name}")
+            out = ctx.mult(_words(s, 4), 1)
+            exp = oracle.g1_scalar_mul(oracle.generator(), sum(a * b for a, b in zip(s, ks)) % R)
+            assert oracle.affine_equal(oracle.projective_to_affine(out[0]), exp), name
+
+
 def test_sort_option_rejected():
     from hardcaml_zprize_amd._lib import ZpError
 
