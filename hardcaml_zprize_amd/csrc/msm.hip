@@ -279,7 +279,7 @@ using namespace zp;
 
 // Per-batch device working memory; one workspace serves one batch at a time.
 struct zp_msm_workspace {
-  size_t cap_K = 0, cap_K2 = 0, cap_K3 = 0, cap_K4 = 0, cap_K5 = 0, cap_Kb = 0;
+  size_t cap_K = 0, cap_K2 = 0, cap_K3 = 0, cap_K4 = 0, cap_K5 = 0;
   size_t cap_entries = 0, cap_units = 0, cap_units2 = 0, cap_groups = 0, cap_gs = 0, cap_win = 0, cap_wsum = 0;
   size_t cap_jobs = 0, cap_pt0 = 0, cap_b0 = 0, cap_scan = 0;
   size_t cap_digits = 0, cap_counts = 0, cap_scanned = 0, cap_segs = 0, cap_tmp = 0, cap_wg0 = 0;
@@ -290,7 +290,7 @@ struct zp_msm_workspace {
   SegDev *segs = nullptr;
   uint32_t *cnt = nullptr, *off = nullptr, *cursor = nullptr, *nunits = nullptr, *ustart = nullptr;
   uint32_t *ubucket = nullptr, *vals = nullptr, *gstart = nullptr, *pt0 = nullptr, *b0 = nullptr;
-  Xyzt *partial = nullptr, *buckets = nullptr, *groups = nullptr, *winsum = nullptr;
+  Xyzt *partial = nullptr, *groups = nullptr, *winsum = nullptr;
   Xyzt *winpart = nullptr;  // slice sums of windows with many groups
   size_t cap_winpart = 0;
   uint32_t *umax = nullptr;  // largest unit count of any bucket (k_units_per_bucket)
@@ -323,8 +323,8 @@ struct zp_msm_workspace {
       arena_cap = c;
       return;
     }
-    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, buckets,
-                  groups, winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
+    void *ps[] = {cnt, off, cursor, nunits, ustart, ubucket, vals, gstart, pt0, b0, partial, groups,
+                  winsum, jobs, wins, scan_tmp, digits, counts, scanned, segs, nc_sums, tmp, win_g0,
                   winpart, umax, fin, d_out, d_consts};
     for (void *p : ps)
       if (p) (void)hipFree(p);
@@ -518,9 +518,10 @@ static int wgrow(zp_msm_workspace &ws, T *&p, size_t &cap, size_t need) {
 
 static uint32_t auto_log_u(size_t entries, int requested) {
   if (requested >= 0) return (uint32_t)requested;
-  // aim for ~2^19 work units; units of 8 .. 256 entries
+  // aim for ~2^19 accumulation slices of 8 .. 1024 entries (longer slices
+  // cut fewer buckets into several units for k_unit_tree to add)
   int lu = 0;
-  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 8) lu++;
+  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 10) lu++;
   return (uint32_t)std::max(lu, 3);
 }
 
@@ -861,7 +862,6 @@ static int plan_batch(zp_msm_workspace &ws, const HostJob *jobs, size_t count, c
   ZP_TRY(wgrow(ws, ws.cursor, ws.cap_K3, K + 1));
   ZP_TRY(wgrow(ws, ws.nunits, ws.cap_K4, K + 1));
   ZP_TRY(wgrow(ws, ws.ustart, ws.cap_K5, K + 1));
-  ZP_TRY(wgrow(ws, ws.buckets, ws.cap_Kb, K + 1));
   ZP_TRY(wgrow(ws, ws.vals, ws.cap_entries, std::max<uint64_t>(entries, 1)));
   ZP_TRY(wgrow(ws, ws.ubucket, ws.cap_units, max_units));
   ZP_TRY(wgrow(ws, ws.partial, ws.cap_units2, max_units));
@@ -1045,14 +1045,15 @@ static int launch_batch(zp_msm_workspace &ws, const BatchPlan &P, hipStream_t st
   // suffice.
   const uint64_t units_per_bucket = ((n_max + (1ull << log_u) - 1) >> log_u) + 1;
   for (uint64_t stride = 1; stride < units_per_bucket; stride <<= 1) {
-    hipLaunchKernelGGL(k_unit_tree, dim3(grid_for(max_units)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ubucket,
+    // (a level with no work returns at once; 2048 blocks keep that cheap)
+    hipLaunchKernelGGL(k_unit_tree, dim3(grid_for(max_units, MSM_THREADS, 2048)), dim3(MSM_THREADS), 0, stream,
+                       ws.partial, ws.ubucket,
                        ws.ustart, ws.nunits, total_units, ws.umax, (uint32_t)stride, KC);
     ZP_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_bucket_sum, dim3(grid_for(K)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.nunits,
-                     (uint32_t)K, ws.buckets, KC);
   ZP_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.buckets, ws.cnt, ws.wins,
+  hipLaunchKernelGGL(k_window_groups, dim3(grid_for(gs)), dim3(MSM_THREADS), 0, stream, ws.partial, ws.ustart, ws.cnt,
+                     ws.wins,
                      ws.gstart, Wt, log_g, ws.groups, KC);
   ZP_HIP(hipGetLastError());
   // Windows of many groups (a shifted-table job's one window) are summed in

@@ -16,7 +16,6 @@
 //                    unit (the FPGA bucket pipeline, top.ml /
 //                    mixed_add_precompute.ml);
 //   k_unit_tree      per bucket: log-depth pairwise sum of its unit partials;
-//   k_bucket_sum     per bucket: the reduced partial (identity if empty);
 //   k_window_groups  per group of g buckets: running-sum reduction
 //                    (driver.cpp:318-339, pippenger.h:25-33) done as
 //                    sum_b (b - lo + 1) S_b + lo * sum_b S_b;
@@ -608,7 +607,11 @@ __global__ void k_units_per_bucket(const uint32_t *off, const uint32_t *cnt, uin
       nunits[k] = v;
     }
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-    if ((threadIdx.x & 63) == 0 && v > 1) atomicMax(umax, v);
+    // most buckets touch one or two slices: read first, so that the waves
+    // whose maximum is already there skip the atomic (a value read late is
+    // only smaller, never larger, than the current one)
+    if ((threadIdx.x & 63) == 0 && v > 1 && v > __hip_atomic_load(umax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(umax, v);
   }
 }
 
@@ -638,8 +641,8 @@ __global__ void k_fill_units(const uint32_t *ustart, uint32_t K, const uint32_t 
 // idle while the wave's largest bucket finishes).  A slice's entries belong
 // to one or more consecutive buckets; each (bucket, slice) piece is a work
 // unit whose partial goes to partial[u], u = ustart[k] + the slice's rank
-// among the slices bucket k touches, and k_unit_tree / k_bucket_sum add a
-// bucket's pieces.  The bucket changes inside the one flat loop (a short
+// among the slices bucket k touches, and k_unit_tree adds a bucket's
+// pieces into its first unit.  The bucket changes inside the one flat loop (a short
 // divergent branch), so lanes never wait for each other's piece lengths.
 __global__ void __launch_bounds__(MSM_THREADS) k_accumulate(BatchDev B, const uint32_t *vals, const uint32_t *off,
                                                             const uint32_t *ustart, uint32_t K, uint32_t log_u,
@@ -697,14 +700,6 @@ __global__ void __launch_bounds__(MSM_THREADS) k_unit_tree(Xyzt *partial, const 
   }
 }
 
-// Bucket value = its first unit's partial after the tree levels (identity if empty).
-__global__ void __launch_bounds__(MSM_THREADS) k_bucket_sum(const Xyzt *partial, const uint32_t *ustart,
-                                                            const uint32_t *nunits, uint32_t K_total,
-                                                            Xyzt *buckets, FConsts K) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K_total; k += gridDim.x * blockDim.x)
-    buckets[k] = nunits[k] ? partial[ustart[k]] : ec_identity(K);
-}
-
 // acc = m * p for a small non-negative integer m (double-and-add, MSB first).
 __device__ __forceinline__ Xyzt ec_mul_small(const Xyzt &p, uint32_t m, const FConsts &K) {
   Xyzt r = ec_identity(K);
@@ -723,10 +718,10 @@ __device__ __forceinline__ Xyzt ec_mul_small(const Xyzt &p, uint32_t m, const FC
 // (cnt = 0) above a group's highest non-empty one are skipped: the running
 // sum is still the identity there (a shifted-table job's shared buckets above
 // the largest digit are all empty).
-__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *buckets, const uint32_t *cnt,
-                                                               const WinDev *wins, const uint32_t *gstart,
-                                                               uint32_t W_total, uint32_t log_g, Xyzt *groups,
-                                                               FConsts K) {
+__global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *partial, const uint32_t *ustart,
+                                                               const uint32_t *cnt, const WinDev *wins,
+                                                               const uint32_t *gstart, uint32_t W_total,
+                                                               uint32_t log_g, Xyzt *groups, FConsts K) {
   const uint32_t total = gstart[W_total];
   for (uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x; gi < total; gi += gridDim.x * blockDim.x) {
     const uint32_t w = upper_index(gstart, W_total, gi);
@@ -736,7 +731,9 @@ __global__ void __launch_bounds__(MSM_THREADS) k_window_groups(const Xyzt *bucke
     while (hi > lo && cnt[wd.base + hi - 1] == 0) hi--;
     Xyzt run = ec_identity(K), acc = ec_identity(K);
     for (int b = (int)hi - 1; b >= (int)lo; b--) {
-      ec_add(run, buckets[wd.base + b], K);
+      // a non-empty bucket's sum is its first unit's partial after the tree
+      // levels; an empty one adds nothing to the running sum
+      if (cnt[wd.base + b]) ec_add(run, partial[ustart[wd.base + b]], K);
       ec_add(acc, run, K);
     }
     // bucket b holds digit magnitude b + 1: weight (b + 1) = (b - lo + 1) + lo
