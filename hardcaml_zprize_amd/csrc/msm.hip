@@ -518,10 +518,11 @@ static int wgrow(zp_msm_workspace &ws, T *&p, size_t &cap, size_t need) {
 
 static uint32_t auto_log_u(size_t entries, int requested) {
   if (requested >= 0) return (uint32_t)requested;
-  // aim for ~2^19 accumulation slices of 8 .. 1024 entries (longer slices
-  // cut fewer buckets into several units for k_unit_tree to add)
+  // aim for ~2^19 accumulation slices of 8 .. 512 entries (longer slices
+  // cut fewer buckets into several units for k_unit_tree to add; at 2^10
+  // entries the 320-job mixed batch was ~2 % slower, the 2^26 MSM equal)
   int lu = 0;
-  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 10) lu++;
+  while ((entries >> (lu + 1)) >= (1u << 19) && lu < 9) lu++;
   return (uint32_t)std::max(lu, 3);
 }
 
