@@ -381,6 +381,8 @@ def bench_batched(args, torch):
     msm_bufs = [zmsm.DeviceBuffer(32 << k).upload(sc) for k, sc in zip(msm_logs, msm_sc)]
     plans = {k: zntt.NttGpuDriver(k, num_buffers=1) for k in set(ntt_logs)}
     table = autotune.load()
+    ntt_tiles = autotune.load_ntt_tiles()
+    autotune.apply_ntt_tiles(ntt_tiles)  # both forms: single-transform tiles (plain), batch-group tiles (graph)
     plain_ws = zmsm.MsmWorkspace()
     s1 = ctypes.c_void_p()
     call("zp_stream_create", ctypes.byref(s1))
@@ -458,6 +460,8 @@ def bench_batched(args, torch):
             "graph": True, "graph_create_s": create_s,
             "autotune_table": os.path.basename(autotune.DEFAULT_PATH) if table else None,
             "tuned_windows": {str(k): v for k, v in sorted(table.items())},
+            "tuned_ntt_tiles": {mode: {str(k): list(v) for k, v in sorted(per.items())}
+                                for mode, per in ntt_tiles.items()},
             "table_options": args.batch_options,
             "checked": "graph replays equal the one-at-a-time results (NTT vectors and MSM bytes); every MSM "
                        "equals (sum s_i k_i mod r) G",
@@ -568,6 +572,7 @@ def device_bound(specs, tables, pool, ntt_pool):
     from hardcaml_zprize_amd._lib import call
 
     windows = autotune.load()
+    autotune.apply_ntt_tiles()
     d_pool = zmsm.DeviceBuffer(pool.nbytes).upload(pool)
     d_ntt = zmsm.DeviceBuffer(ntt_pool.nbytes).upload(ntt_pool)
     ws = zmsm.MsmWorkspace()

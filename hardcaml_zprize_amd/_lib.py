@@ -37,6 +37,8 @@ SIGNATURES = {
     "zp_ntt_batch_create": (_i32, [_vp, _vp, _vp, _sz, ctypes.POINTER(_vp)]),
     "zp_ntt_batch_launch": (_i32, [_vp, _i32, _vp]),
     "zp_ntt_batch_destroy": (_i32, [_vp]),
+    "zp_ntt_set_tiles": (_i32, [_u32, _i32, _i32, _i32, _i32]),
+    "zp_ntt_get_tiles": (_i32, [_u32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "msm_init": (_vp, [ctypes.c_char_p, ctypes.c_ssize_t, _vp, ctypes.c_ssize_t]),
     "msm_mult": (None, [_vp, _vp, _u64, _vp]),
     "msm_free": (None, [_vp]),

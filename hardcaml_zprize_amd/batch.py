@@ -6,7 +6,8 @@
 and buckets concatenated) are two parallel branches of one graph, whose
 last node copies the MSM results to pinned host memory.  `launch()` replays
 the graph (asynchronous), `wait()` returns the MSM results.  Per-job MSM
-window widths come from the autotuner's table (`autotune.window_for`).
+window widths and the NTT groups' tiles come from the autotuner's table
+(`autotune.window_for`, `autotune.apply_ntt_tiles`).
 
 No reference counterpart: the reference runs one transform or one MSM at a
 time on its device (zprize/ntt/host/bench_throughput.cpp:61-107,
@@ -29,11 +30,13 @@ class MixedBatch:
     """ntt: list of (device_ptr, log_n, direction); msm: list of
     (MsmContext, n, d_scalars_ptr[, window]).  window 0 or absent: the
     autotuner's width for the job's size class (0 = library default) when
-    `tuned`, else the library default."""
+    `tuned`, else the library default.  `tuned` also sets the library's NTT
+    tiles from the autotuner's table before the graph is captured."""
 
     def __init__(self, ntt, msm, options="", tuned=True, table=None):
         self.out = np.zeros(max(len(msm), 1), dtype=G1_PROJ_DTYPE)
         windows = autotune.load() if (tuned and table is None) else (table or {})
+        self.ntt_tiles_applied = autotune.apply_ntt_tiles() if tuned else 0
         self.windows = []
         jobs = (MsmJob * max(len(msm), 1))()
         base = self.out.ctypes.data

@@ -108,11 +108,25 @@ static int make_tables(unsigned log_n, NttTables **out) {
 // ------------------------------------------------------------ dispatch --
 struct NttShape {
   unsigned log_n, log_n1, log_n2;
+  unsigned log_t1 = 0, log_t2 = 0;  // LDS passes: log2 sequences per workgroup
 };
+
+// Tile choices of the two-pass LDS transform per size class (2^10 .. 2^20
+// points): the column length 2^log_n1 and the sequences per workgroup of each
+// pass, one table for single transforms and one for the groups of a batch
+// (many jobs on grid.y).  Fields < 0 take the built-in policy.  Set through
+// zp_ntt_set_tiles (the autotuner's table); a transform reads its shape once,
+// before its first pass.
+struct NttTileChoice {
+  int log_n1 = -1, log_t1 = -1, log_t2 = -1;
+};
+static constexpr unsigned TILE_MIN_LOG = 10, TILE_MAX_LOG = 20;
+static std::mutex g_tile_mu;
+static NttTileChoice g_tile_choice[2][TILE_MAX_LOG + 1];
 
 // Up to 2^9 points one workgroup does the whole transform in LDS; above, the
 // two passes spread the work over many workgroups.
-static NttShape shape_of(unsigned log_n) {
+static NttShape shape_of(unsigned log_n, bool batched = false) {
   NttShape s;
   s.log_n = log_n;
   if (log_n <= 9) {
@@ -122,8 +136,15 @@ static NttShape shape_of(unsigned log_n) {
     s.log_n1 = 12;  // 4096-point columns in registers (pass 1)
     s.log_n2 = log_n - 12;
   } else {
-    s.log_n1 = (log_n + 1) / 2;
+    NttTileChoice c;
+    {
+      std::lock_guard<std::mutex> lk(g_tile_mu);
+      c = g_tile_choice[batched ? 1 : 0][log_n];
+    }
+    s.log_n1 = c.log_n1 >= 0 ? (unsigned)c.log_n1 : (log_n + 1) / 2;
     s.log_n2 = log_n - s.log_n1;
+    s.log_t1 = c.log_t1 >= 0 ? (unsigned)c.log_t1 : dft_log_T(s.log_n1, s.log_n2);
+    s.log_t2 = c.log_t2 >= 0 ? (unsigned)c.log_t2 : dft_log_T(s.log_n2, s.log_n1);
   }
   return s;
 }
@@ -288,9 +309,8 @@ static void tile_widths() {
 // data -> scratch; pass 2 = row DFTs, scratch -> data transposed).  A length
 // that is a single DFT (log_n <= 9) has only pass 1, which works in place.
 // scratch must hold n elements otherwise (needs_scratch).
-static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir, int pass,
-                       hipStream_t stream, const JobTabs &jt = JobTabs()) {
-  const NttShape sh = shape_of(t->log_n);
+static int launch_pass(const NttTables *t, const NttShape &sh, uint64_t *d_data, uint64_t *d_scratch, int dir,
+                       int pass, hipStream_t stream, const JobTabs &jt = JobTabs()) {
   if (sh.log_n1 == 12 && sh.log_n2 == 12) {
     tile_widths();
     if (pass == 1)
@@ -336,7 +356,7 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
     p1.in = d_data;
     p1.out = d_scratch;
     p1.log_L = sh.log_n1;
-    p1.log_T = dft_log_T(sh.log_n1, sh.log_n2);
+    p1.log_T = sh.log_t1;
     p1.in_elem_stride = n2;
     p1.in_seq_stride = 1;
     p1.out_elem_stride = n2;
@@ -354,7 +374,7 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
   p2.in = d_scratch;
   p2.out = d_data;
   p2.log_L = sh.log_n2;
-  p2.log_T = dft_log_T(sh.log_n2, sh.log_n1);
+  p2.log_T = sh.log_t2;
   p2.in_elem_stride = 1;
   p2.in_seq_stride = n2;
   p2.out_elem_stride = n1;
@@ -370,8 +390,9 @@ static int launch_pass(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch
 // Enqueue one in-place transform.
 static int launch_transform(const NttTables *t, uint64_t *d_data, uint64_t *d_scratch, int dir,
                             hipStream_t stream, const JobTabs &jt = JobTabs()) {
-  ZP_TRY(launch_pass(t, d_data, d_scratch, dir, 1, stream, jt));
-  return launch_pass(t, d_data, d_scratch, dir, 2, stream, jt);
+  const NttShape sh = shape_of(t->log_n, jt.data != nullptr);
+  ZP_TRY(launch_pass(t, sh, d_data, d_scratch, dir, 1, stream, jt));
+  return launch_pass(t, sh, d_data, d_scratch, dir, 2, stream, jt);
 }
 
 }  // namespace zp
@@ -409,14 +430,16 @@ struct zp_ntt_plan {
 };
 
 // One pass (1 or 2) of the plan's operation on d_data (in place, scratch in between).
-static int plan_pass(const zp_ntt_plan *p, uint64_t *d_data, int dir, int pass, hipStream_t stream) {
+static int plan_pass(const zp_ntt_plan *p, const NttShape &sh, uint64_t *d_data, int dir, int pass,
+                     hipStream_t stream) {
   if (p->reverse) return launch_reverse_phase(p->tables, d_data, p->d_scratch, pass, stream);
-  return launch_pass(p->tables, d_data, p->d_scratch, dir, pass, stream);
+  return launch_pass(p->tables, sh, d_data, p->d_scratch, dir, pass, stream);
 }
 
 static int plan_launch(const zp_ntt_plan *p, uint64_t *d_data, int dir, hipStream_t stream) {
-  ZP_TRY(plan_pass(p, d_data, dir, 1, stream));
-  return plan_pass(p, d_data, dir, 2, stream);
+  const NttShape sh = shape_of(p->tables->log_n);
+  ZP_TRY(plan_pass(p, sh, d_data, dir, 1, stream));
+  return plan_pass(p, sh, d_data, dir, 2, stream);
 }
 
 static int check_dir(const zp_ntt_plan *p, int dir, const char *who) {
@@ -427,6 +450,37 @@ static int check_dir(const zp_ntt_plan *p, int dir, const char *who) {
 
 extern "C" const char *zp_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char *zp_version(void) { return "zpgpu gfx950"; }
+
+extern "C" int zp_ntt_set_tiles(unsigned log_n, int batched, int log_n1, int log_t1, int log_t2) {
+  ZP_CHECK(log_n >= TILE_MIN_LOG && log_n <= TILE_MAX_LOG, ZP_ERR_INVALID_ARG,
+           "zp_ntt_set_tiles: log_n %u not in [%u, %u]", log_n, TILE_MIN_LOG, TILE_MAX_LOG);
+  ZP_CHECK(batched == 0 || batched == 1, ZP_ERR_INVALID_ARG, "zp_ntt_set_tiles: batched %d not 0/1", batched);
+  ZP_CHECK(log_n1 >= -1 && log_t1 >= -1 && log_t2 >= -1, ZP_ERR_INVALID_ARG,
+           "zp_ntt_set_tiles: fields must be >= -1 (got %d, %d, %d)", log_n1, log_t1, log_t2);
+  const int ln = (int)log_n;
+  const int n1 = log_n1 >= 0 ? log_n1 : (ln + 1) / 2, n2 = ln - n1;
+  ZP_CHECK(n1 >= 1 && n1 <= 12 && n2 >= 1 && n2 <= 12, ZP_ERR_INVALID_ARG,
+           "zp_ntt_set_tiles: 2^%d = 2^%d x 2^%d: both sub-DFT lengths must be 2^1 .. 2^12", ln, n1, n2);
+  ZP_CHECK(log_t1 <= std::min(13 - n1, n2), ZP_ERR_INVALID_ARG,
+           "zp_ntt_set_tiles: log_t1 %d > min(13 - %d, %d) (LDS holds 2^13 elements)", log_t1, n1, n2);
+  ZP_CHECK(log_t2 <= std::min(13 - n2, n1), ZP_ERR_INVALID_ARG,
+           "zp_ntt_set_tiles: log_t2 %d > min(13 - %d, %d) (LDS holds 2^13 elements)", log_t2, n2, n1);
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  g_tile_choice[batched][log_n] = NttTileChoice{log_n1, log_t1, log_t2};
+  return ZP_OK;
+}
+
+extern "C" int zp_ntt_get_tiles(unsigned log_n, int batched, int *log_n1, int *log_t1, int *log_t2) {
+  ZP_CHECK(log_n >= TILE_MIN_LOG && log_n <= TILE_MAX_LOG, ZP_ERR_INVALID_ARG,
+           "zp_ntt_get_tiles: log_n %u not in [%u, %u]", log_n, TILE_MIN_LOG, TILE_MAX_LOG);
+  ZP_CHECK(batched == 0 || batched == 1, ZP_ERR_INVALID_ARG, "zp_ntt_get_tiles: batched %d not 0/1", batched);
+  ZP_CHECK(log_n1 && log_t1 && log_t2, ZP_ERR_INVALID_ARG, "zp_ntt_get_tiles: NULL output");
+  const NttShape sh = shape_of(log_n, batched != 0);
+  *log_n1 = (int)sh.log_n1;
+  *log_t1 = (int)sh.log_t1;
+  *log_t2 = (int)sh.log_t2;
+  return ZP_OK;
+}
 
 static int create_plan(unsigned log_n, unsigned num_buffers, bool reverse, zp_ntt_plan **out, const char *who) {
   ZP_CHECK(out != nullptr, ZP_ERR_INVALID_ARG, "%s: out is NULL", who);
@@ -512,11 +566,12 @@ extern "C" int zp_ntt_time_passes(zp_ntt_plan *p, uint64_t *d_data, int dir, int
   std::vector<hipEvent_t> ev(3 * (size_t)reps);
   for (auto &e : ev) ZP_HIP(hipEventCreate(&e));
   int rc = ZP_OK;
+  const NttShape sh = shape_of(p->tables->log_n);
   for (int r = 0; r < reps && rc == ZP_OK; r++) {
     (void)hipEventRecord(ev[3 * r], s);
-    rc = plan_pass(p, d_data, dir, 1, s);
+    rc = plan_pass(p, sh, d_data, dir, 1, s);
     (void)hipEventRecord(ev[3 * r + 1], s);
-    if (rc == ZP_OK) rc = plan_pass(p, d_data, dir, 2, s);
+    if (rc == ZP_OK) rc = plan_pass(p, sh, d_data, dir, 2, s);
     (void)hipEventRecord(ev[3 * r + 2], s);
   }
   double t1 = 0, t2 = 0;

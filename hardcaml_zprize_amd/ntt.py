@@ -137,6 +137,36 @@ def ntt_batch_device(ptrs, log_ns, dirs, stream=None):
     call("zp_ntt_batch_device", P, L, D, n, ctypes.c_void_p(stream or 0))
 
 
+TILE_MIN_LOG, TILE_MAX_LOG = 10, 20
+
+
+def set_tiles(log_n, batched, log_n1=-1, log_t1=-1, log_t2=-1):
+    """Tile choice of one size class of the two-pass LDS transform (log_n in
+    [10, 20]) for single transforms (batched=False) or batch groups
+    (batched=True): column length 2^log_n1 and log2 sequences per workgroup
+    of each pass; -1 = the built-in policy.  Raises ZpError on an invalid
+    choice (sub-DFT lengths outside 2^1..2^12, a tile beyond 2^13 elements)."""
+    call("zp_ntt_set_tiles", int(log_n), 1 if batched else 0, int(log_n1), int(log_t1), int(log_t2))
+
+
+def get_tiles(log_n, batched):
+    """Effective (log_n1, log_t1, log_t2) of a size class."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    call("zp_ntt_get_tiles", int(log_n), 1 if batched else 0, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def tile_candidates(log_n):
+    """Every valid (log_n1, log_t1, log_t2) of a size class."""
+    out = []
+    for n1 in range(max(1, log_n - 12), min(12, log_n - 1) + 1):
+        n2 = log_n - n1
+        for t1 in range(0, min(13 - n1, n2) + 1):
+            for t2 in range(0, min(13 - n2, n1) + 1):
+                out.append((n1, t1, t2))
+    return out
+
+
 class NttBatch:
     """Many in-place transforms of mixed sizes/directions, grouped into one
     launch per (size, direction, pass); optionally replayed from a HIP graph."""
@@ -165,4 +195,5 @@ class NttBatch:
             pass
 
 
-__all__ = ["NttBatch", "NttGpuDriver", "UserBuffer", "ntt_batch_device", "FORWARD", "INVERSE", "ZpError"]
+__all__ = ["NttBatch", "NttGpuDriver", "UserBuffer", "ntt_batch_device", "set_tiles", "get_tiles", "tile_candidates",
+           "FORWARD", "INVERSE", "ZpError"]

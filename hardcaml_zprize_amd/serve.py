@@ -141,6 +141,7 @@ class GpuServer:
         self.failed = None  # set when a worker could not start; the server then refuses work
         self.batches = 0
         self.windows = autotune.load()  # size class -> tuned MSM window (empty = library default)
+        self.ntt_tiles_applied = autotune.apply_ntt_tiles()  # NTT tiles per size class (process-wide)
         self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
         for t in self.threads:
             t.start()

@@ -123,6 +123,21 @@ int zp_ntt_batch_destroy(zp_ntt_batch *batch);
 int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
                         void *stream);
 
+/* Tile choices of one size class of the two-pass LDS transform (log_n in
+ * [10, 20]), for single transforms (batched = 0: plans, the buffer API,
+ * zp_ntt_inplace_device) or for the groups of a batch (batched = 1:
+ * zp_ntt_batch_*, zp_mixed_batch_*): the column length 2^log_n1 (pass 1
+ * sub-DFTs; log_n1 and log_n - log_n1 in [1, 12]) and log2 of the sequences
+ * per workgroup of pass 1 (log_t1 <= min(13 - log_n1, log_n - log_n1)) and
+ * pass 2 (log_t2 <= min(13 - (log_n - log_n1), log_n1)).  A field of -1
+ * takes the built-in policy; (-1, -1, -1) restores it.  Process-wide; takes
+ * effect for transforms launched (or batches captured) afterwards.  No
+ * reference counterpart (the FPGA core's layout is fixed): this is the hook
+ * of the size-class autotuner (hardcaml_zprize_amd/autotune.py).
+ * zp_ntt_get_tiles returns the effective choice (built-in fields resolved). */
+int zp_ntt_set_tiles(unsigned log_n, int batched, int log_n1, int log_t1, int log_t2);
+int zp_ntt_get_tiles(unsigned log_n, int batched, int *log_n1, int *log_t1, int *log_t2);
+
 /* ------------------------------------------------------------------ MSM --
  * Layouts of zprize/msm_pippenger/host/msm_compute/include/bls12_377_g1/rust_types.h:8-83.
  * Base-field words are little-endian 64-bit limbs in Montgomery form with

@@ -1,0 +1,76 @@
+"""Every tile choice of the two-pass LDS transform (zp_ntt_set_tiles: the
+column length and the sequences per workgroup of each pass) gives the
+oracle's output, forward and inverse, for single transforms and for batch
+groups (jobs on grid.y, replayed from a graph)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from hardcaml_zprize_amd import msm as zmsm
+from hardcaml_zprize_amd import ntt as zntt
+from hardcaml_zprize_amd._lib import call
+
+pytestmark = pytest.mark.gpu
+P = oracle.P
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(autouse=True)
+def _builtin_policy():
+    yield
+    for lg in range(zntt.TILE_MIN_LOG, zntt.TILE_MAX_LOG + 1):
+        for b in (False, True):
+            zntt.set_tiles(lg, b)
+
+
+def _rand(n, seed):
+    return np.random.default_rng(seed).integers(0, P, size=n, dtype=np.uint64)
+
+
+@pytest.mark.parametrize("log_n", [10, 13, 16, 20])
+def test_every_tile_single(log_n):
+    a = _rand(1 << log_n, 500 + log_n)
+    fwd, inv = oracle.ntt(a), oracle.ntt(a, inverse=True)
+    cands = zntt.tile_candidates(log_n)
+    _say(f"ntt tiles single log_n={log_n}: {len(cands)} choices")
+    with zntt.NttGpuDriver(log_n) as d:
+        for c in cands:
+            zntt.set_tiles(log_n, False, *c)
+            assert zntt.get_tiles(log_n, False) == c
+            assert np.array_equal(d.simple_evaluate(a, zntt.FORWARD), fwd), c
+            assert np.array_equal(d.simple_evaluate(a, zntt.INVERSE), inv), c
+
+
+@pytest.mark.parametrize("log_n,stride", [(10, 1), (14, 3), (17, 2), (20, 1)])
+def test_tiles_batched(log_n, stride):
+    """Three jobs of one size (two forward, one inverse) per choice; the graph
+    is captured after the choice is set and replayed twice."""
+    xs = [_rand(1 << log_n, 700 + 3 * log_n + i) for i in range(3)]
+    dirs = [0, 1, 0]
+    want = [oracle.ntt(x, inverse=bool(d)) for x, d in zip(xs, dirs)]
+    bufs = [zmsm.DeviceBuffer(8 << log_n) for _ in xs]
+    cands = zntt.tile_candidates(log_n)[::stride]
+    _say(f"ntt tiles batched log_n={log_n}: {len(cands)} choices")
+    try:
+        for c in cands:
+            zntt.set_tiles(log_n, True, *c)
+            b = zntt.NttBatch([x.ptr for x in bufs], [log_n] * 3, dirs)
+            try:
+                for _ in range(2):
+                    for buf, x in zip(bufs, xs):
+                        buf.upload(x)
+                    b.launch(use_graph=True)
+                    call("zp_device_synchronize")
+                    for buf, w in zip(bufs, want):
+                        assert np.array_equal(buf.download(np.empty(1 << log_n, dtype=np.uint64)), w), c
+            finally:
+                b.close()
+    finally:
+        for buf in bufs:
+            buf.free()
