@@ -74,3 +74,21 @@ def test_tiles_batched(log_n, stride):
     finally:
         for buf in bufs:
             buf.free()
+
+
+def test_autotuner_ntt_leaves_valid_applied_choices():
+    """The tuner's NTT search on two size classes: every chosen tile is a
+    valid one, it is what the library then uses, and a transform under it
+    still equals the oracle."""
+    from hardcaml_zprize_amd import autotune
+
+    _say("ntt autotuner search 2^11, 2^15")
+    res = autotune.tune_ntt([11, 15], batch_jobs=4)
+    for mode, per in res.items():
+        for lg, (choice, times) in per.items():
+            assert choice in zntt.tile_candidates(lg)
+            assert zntt.get_tiles(lg, mode == "batched") == choice
+            assert "%d,%d,%d" % choice in times and len(times) >= 3
+    a = _rand(1 << 15, 99)
+    with zntt.NttGpuDriver(15) as d:
+        assert np.array_equal(d.simple_evaluate(a, zntt.FORWARD), oracle.ntt(a))
