@@ -2,11 +2,12 @@
 
 MSM: for each size class log2 N it times `window` candidates around the
 library's default policy on the GPU and keeps the fastest.  NTT: for each
-size class of the two-pass LDS transform (2^10 .. 2^20 points) it chooses
-the column length and the sequences per workgroup of both passes
-(zp_ntt_set_tiles), once for single transforms and once for the groups of a
-batch (B jobs of one size on grid.y), by coordinate descent: the split among
-the balanced one +- 1, then pass 1's tile, then pass 2's.  The result is a
+size class of the two-pass transform (2^10 .. 2^23 points) it chooses the
+column length (4096: the register-resident column kernel) and the sequences
+per workgroup of both LDS passes (zp_ntt_set_tiles), once for single
+transforms and once for the groups of a batch (B jobs of one size on
+grid.y), by coordinate descent: the split among the built-in one +- 1, then
+pass 1's tile, then pass 2's.  The result is a
 small JSON table (`autotune.json`) that `MixedBatch`, the serving loop and
 the bench apply (`apply_ntt_tiles`, per-job MSM `window`).  `ZP_NTT_TILE`
 (4 or 8) selects the 64 x 64 register kernels' tile for 2^21 and above (set,
@@ -188,7 +189,7 @@ def main():
                     help="msm_init options of the table tuned on (the batched form's shared table)")
     ap.add_argument("--out", default=DEFAULT_PATH)
     ap.add_argument("--ntt-min-log", type=int, default=10)
-    ap.add_argument("--ntt-max-log", type=int, default=20)
+    ap.add_argument("--ntt-max-log", type=int, default=23)
     ap.add_argument("--ntt-batch-jobs", type=int, default=12,
                     help="jobs per size class in the batched NTT timing (the batched form's ~256/11/2 per group)")
     ap.add_argument("--ntt-only", action="store_true", help="tune the NTT tiles only; keep the MSM table")

@@ -111,16 +111,19 @@ struct NttShape {
   unsigned log_t1 = 0, log_t2 = 0;  // LDS passes: log2 sequences per workgroup
 };
 
-// Tile choices of the two-pass LDS transform per size class (2^10 .. 2^20
-// points): the column length 2^log_n1 and the sequences per workgroup of each
-// pass, one table for single transforms and one for the groups of a batch
-// (many jobs on grid.y).  Fields < 0 take the built-in policy.  Set through
-// zp_ntt_set_tiles (the autotuner's table); a transform reads its shape once,
-// before its first pass.
+// Tile choices per size class (2^10 .. 2^23 points): the column length
+// 2^log_n1 (4096: pass 1 on the register-resident column kernel; shorter:
+// on the LDS kernels) and the sequences per workgroup of each LDS pass, one
+// table for single transforms and one for the groups of a batch (many jobs
+// on grid.y).  Fields < 0 take the built-in policy (balanced split up to
+// 2^20, 4096-point columns from 2^21).  Set through zp_ntt_set_tiles (the
+// autotuner's table); a transform reads its shape once, before its first
+// pass.
 struct NttTileChoice {
   int log_n1 = -1, log_t1 = -1, log_t2 = -1;
 };
-static constexpr unsigned TILE_MIN_LOG = 10, TILE_MAX_LOG = 20;
+static constexpr unsigned TILE_MIN_LOG = 10, TILE_MAX_LOG = 23;
+static unsigned default_log_n1(unsigned log_n) { return log_n >= 21 ? 12 : (log_n + 1) / 2; }
 static std::mutex g_tile_mu;
 static NttTileChoice g_tile_choice[2][TILE_MAX_LOG + 1];
 
@@ -132,8 +135,8 @@ static NttShape shape_of(unsigned log_n, bool batched = false) {
   if (log_n <= 9) {
     s.log_n1 = log_n;
     s.log_n2 = 0;
-  } else if (log_n >= 21) {
-    s.log_n1 = 12;  // 4096-point columns in registers (pass 1)
+  } else if (log_n > TILE_MAX_LOG) {
+    s.log_n1 = 12;  // 4096 x 4096: register-resident kernels for both passes
     s.log_n2 = log_n - 12;
   } else {
     NttTileChoice c;
@@ -141,7 +144,7 @@ static NttShape shape_of(unsigned log_n, bool batched = false) {
       std::lock_guard<std::mutex> lk(g_tile_mu);
       c = g_tile_choice[batched ? 1 : 0][log_n];
     }
-    s.log_n1 = c.log_n1 >= 0 ? (unsigned)c.log_n1 : (log_n + 1) / 2;
+    s.log_n1 = c.log_n1 >= 0 ? (unsigned)c.log_n1 : default_log_n1(log_n);
     s.log_n2 = log_n - s.log_n1;
     s.log_t1 = c.log_t1 >= 0 ? (unsigned)c.log_t1 : dft_log_T(s.log_n1, s.log_n2);
     s.log_t2 = c.log_t2 >= 0 ? (unsigned)c.log_t2 : dft_log_T(s.log_n2, s.log_n1);
@@ -350,6 +353,8 @@ static int launch_pass(const NttTables *t, const NttShape &sh, uint64_t *d_data,
     a.out_elem_stride = 1;
     a.out_seq_stride = n;
     a.tw_sub = t->d_sub[dir];
+    a.tw_full = t->d_full[dir];
+    a.dir = dir;
     a.scale = scale;
     a.in_tab = jt.data;
     a.out_tab = jt.data;
@@ -368,6 +373,8 @@ static int launch_pass(const NttTables *t, const NttShape &sh, uint64_t *d_data,
     p1.out_elem_stride = n2;
     p1.out_seq_stride = 1;
     p1.tw_sub = t->d_sub[dir];
+    p1.tw_full = t->d_full[dir];
+    p1.dir = dir;
     p1.tw_lo = t->d_lo[dir];
     p1.tw_hi = t->d_hi[dir];
     p1.scale = 1;
@@ -386,6 +393,8 @@ static int launch_pass(const NttTables *t, const NttShape &sh, uint64_t *d_data,
   p2.out_elem_stride = n1;
   p2.out_seq_stride = 1;
   p2.tw_sub = t->d_sub[dir];
+  p2.tw_full = t->d_full[dir];
+  p2.dir = dir;
   // after the register-resident 4096-point columns, n^-1 is already applied
   p2.scale = (sh.log_n1 == 12 && sh.log_n2 >= 3) ? 1 : scale;
   p2.in_tab = jt.scratch;
@@ -464,7 +473,7 @@ extern "C" int zp_ntt_set_tiles(unsigned log_n, int batched, int log_n1, int log
   ZP_CHECK(log_n1 >= -1 && log_t1 >= -1 && log_t2 >= -1, ZP_ERR_INVALID_ARG,
            "zp_ntt_set_tiles: fields must be >= -1 (got %d, %d, %d)", log_n1, log_t1, log_t2);
   const int ln = (int)log_n;
-  const int n1 = log_n1 >= 0 ? log_n1 : (ln + 1) / 2, n2 = ln - n1;
+  const int n1 = log_n1 >= 0 ? log_n1 : (int)default_log_n1(log_n), n2 = ln - n1;
   ZP_CHECK(n1 >= 1 && n1 <= 12 && n2 >= 1 && n2 <= 12, ZP_ERR_INVALID_ARG,
            "zp_ntt_set_tiles: 2^%d = 2^%d x 2^%d: both sub-DFT lengths must be 2^1 .. 2^12", ln, n1, n2);
   ZP_CHECK(log_t1 <= std::min(13 - n1, n2), ZP_ERR_INVALID_ARG,

@@ -137,15 +137,16 @@ def ntt_batch_device(ptrs, log_ns, dirs, stream=None):
     call("zp_ntt_batch_device", P, L, D, n, ctypes.c_void_p(stream or 0))
 
 
-TILE_MIN_LOG, TILE_MAX_LOG = 10, 20
+TILE_MIN_LOG, TILE_MAX_LOG = 10, 23
 
 
 def set_tiles(log_n, batched, log_n1=-1, log_t1=-1, log_t2=-1):
-    """Tile choice of one size class of the two-pass LDS transform (log_n in
-    [10, 20]) for single transforms (batched=False) or batch groups
-    (batched=True): column length 2^log_n1 and log2 sequences per workgroup
-    of each pass; -1 = the built-in policy.  Raises ZpError on an invalid
-    choice (sub-DFT lengths outside 2^1..2^12, a tile beyond 2^13 elements)."""
+    """Tile choice of one size class (log_n in [10, 23]) for single transforms
+    (batched=False) or batch groups (batched=True): column length 2^log_n1
+    (2^12: pass 1 on the register-resident column kernel) and log2 sequences
+    per workgroup of each pass; -1 = the built-in policy.  Raises ZpError on
+    an invalid choice (sub-DFT lengths outside 2^1..2^12, a tile beyond 2^13
+    elements)."""
     call("zp_ntt_set_tiles", int(log_n), 1 if batched else 0, int(log_n1), int(log_t1), int(log_t2))
 
 

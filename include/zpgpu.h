@@ -123,14 +123,16 @@ int zp_ntt_batch_destroy(zp_ntt_batch *batch);
 int zp_ntt_batch_device(uint64_t *const *d_ptrs, const unsigned *log_ns, const int *dirs, size_t count,
                         void *stream);
 
-/* Tile choices of one size class of the two-pass LDS transform (log_n in
- * [10, 20]), for single transforms (batched = 0: plans, the buffer API,
- * zp_ntt_inplace_device) or for the groups of a batch (batched = 1:
- * zp_ntt_batch_*, zp_mixed_batch_*): the column length 2^log_n1 (pass 1
- * sub-DFTs; log_n1 and log_n - log_n1 in [1, 12]) and log2 of the sequences
- * per workgroup of pass 1 (log_t1 <= min(13 - log_n1, log_n - log_n1)) and
- * pass 2 (log_t2 <= min(13 - (log_n - log_n1), log_n1)).  A field of -1
- * takes the built-in policy; (-1, -1, -1) restores it.  Process-wide; takes
+/* Tile choices of one size class (log_n in [10, 23]), for single transforms
+ * (batched = 0: plans, the buffer API, zp_ntt_inplace_device) or for the
+ * groups of a batch (batched = 1: zp_ntt_batch_*, zp_mixed_batch_*): the
+ * column length 2^log_n1 (pass 1 sub-DFTs; log_n1 and log_n - log_n1 in
+ * [1, 12]; 12 puts pass 1 on the register-resident 4096-point column kernel)
+ * and log2 of the sequences per workgroup of pass 1 (log_t1 <= min(13 -
+ * log_n1, log_n - log_n1); unused when log_n1 is 12) and pass 2 (log_t2 <=
+ * min(13 - (log_n - log_n1), log_n1)).  A field of -1 takes the built-in
+ * policy (balanced split up to 2^20, log_n1 = 12 from 2^21); (-1, -1, -1)
+ * restores it.  Process-wide; takes
  * effect for transforms launched (or batches captured) afterwards.  No
  * reference counterpart (the FPGA core's layout is fixed): this is the hook
  * of the size-class autotuner (hardcaml_zprize_amd/autotune.py).

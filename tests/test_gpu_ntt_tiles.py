@@ -33,7 +33,7 @@ def _rand(n, seed):
     return np.random.default_rng(seed).integers(0, P, size=n, dtype=np.uint64)
 
 
-@pytest.mark.parametrize("log_n", [10, 13, 16, 20])
+@pytest.mark.parametrize("log_n", [10, 13, 16, 20, 21, 22, 23])
 def test_every_tile_single(log_n):
     a = _rand(1 << log_n, 500 + log_n)
     fwd, inv = oracle.ntt(a), oracle.ntt(a, inverse=True)
@@ -47,7 +47,7 @@ def test_every_tile_single(log_n):
             assert np.array_equal(d.simple_evaluate(a, zntt.INVERSE), inv), c
 
 
-@pytest.mark.parametrize("log_n,stride", [(10, 1), (14, 3), (17, 2), (20, 1)])
+@pytest.mark.parametrize("log_n,stride", [(10, 1), (14, 3), (17, 2), (20, 1), (22, 1)])
 def test_tiles_batched(log_n, stride):
     """Three jobs of one size (two forward, one inverse) per choice; the graph
     is captured after the choice is set and replayed twice."""

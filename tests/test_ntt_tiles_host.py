@@ -21,9 +21,10 @@ def _builtin_policy():
 
 
 def builtin(log_n):
-    """The library's built-in choice restated: balanced split, sequences per
-    workgroup up to 2^13 LDS elements and few enough to keep 2^8 workgroups."""
-    n1 = (log_n + 1) // 2
+    """The library's built-in choice restated: balanced split (4096-point
+    columns from 2^21), sequences per workgroup up to 2^13 LDS elements and few
+    enough to keep 2^8 workgroups."""
+    n1 = 12 if log_n > 20 else (log_n + 1) // 2
     n2 = log_n - n1
 
     def t(log_l, log_seq):
@@ -34,14 +35,14 @@ def builtin(log_n):
     return n1, t(n1, n2), t(n2, n1)
 
 
-@pytest.mark.parametrize("log_n", range(10, 21))
+@pytest.mark.parametrize("log_n", range(10, 24))
 def test_builtin_policy(log_n):
     for b in (False, True):
         assert zntt.get_tiles(log_n, b) == builtin(log_n)
 
 
 def test_candidates_are_exactly_the_accepted_choices():
-    for lg in (10, 15, 20):
+    for lg in (10, 15, 20, 22):
         cands = set(zntt.tile_candidates(lg))
         for n1 in range(0, 14):
             for t1 in range(0, 15):
@@ -68,9 +69,10 @@ def test_set_get_and_modes_are_separate():
     assert zntt.get_tiles(16, False) == builtin(16)
 
 
-@pytest.mark.parametrize("args", [(9, 0, -1, -1, -1), (21, 0, -1, -1, -1), (16, 2, -1, -1, -1),
+@pytest.mark.parametrize("args", [(9, 0, -1, -1, -1), (16, 2, -1, -1, -1),
                                   (16, 0, -2, -1, -1), (16, 0, 13, -1, -1), (16, 0, 3, -1, -1),
-                                  (16, 0, 8, 6, -1), (16, 0, 8, -1, 6), (20, 1, 8, -1, 6)])
+                                  (16, 0, 8, 6, -1), (16, 0, 8, -1, 6), (20, 1, 8, -1, 6),
+                                  (21, 0, 8, -1, -1), (22, 1, 12, 2, -1), (23, 0, -1, -1, 3), (24, 0, -1, -1, -1)])
 def test_invalid_choices_rejected(args):
     with pytest.raises(ZpError):
         call("zp_ntt_set_tiles", *args)
