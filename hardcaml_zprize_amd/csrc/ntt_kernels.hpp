@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft_lds(DftArgs a) {
   const unsigned L = 1u << a.log_L;
   const unsigned T = 1u << a.log_T;
   const unsigned total = L << a.log_T;
-  const uint64_t seq0 = (uint64_t)blockIdx.x << a.log_T;
+  const uint64_t seq0 = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) << a.log_T;  // adjacent tiles on one XCD
   const bool in_rows = a.in_elem_stride == 1;
   for (unsigned idx = threadIdx.x; idx < total; idx += DFT_THREADS) {
     unsigned s, i;
