@@ -130,13 +130,14 @@ __global__ void __launch_bounds__(DFT_THREADS) k_dft_lds(DftArgs a) {
 // A sequence sits in LDS as a 3-d array, element (i1, i2, i3) at
 // i1 + A i2 + A B i3 (input j lands at its natural index; padded, reg_pad);
 // stages 1 and 2 transform their lines in place, stage 3 writes X[k] to slot
-// k.  A work item is 16
-// values: 16 / R lines of a radix-R stage, line q L / 16 + t for item t, so
-// that the lanes of a wave take consecutive lines.  The radix-R DFTs use only
-// power-of-two twiddles (every root of order <= 64 is one) and the lazy
-// arithmetic of the 4096-point kernels; the two twiddle layers are general
-// products with w_4096^i from a table.  Inputs >= p count as their residue
-// (the lazy stages accept any u64), outputs are canonical.
+// k.  A work item (one thread) is 16 values: 16 / R lines of a radix-R
+// stage, line q L / 16 + t for item t, so that the lanes of a wave take
+// consecutive lines.  The radix-R DFTs use only power-of-two twiddles (every
+// root of order <= 64 is one) and the lazy arithmetic of the 4096-point
+// kernels; the two twiddle layers are general products with w_4096^i from a
+// table.  Inputs >= p count as their residue (the lazy stages accept any
+// u64), outputs are canonical.
+
 // Threads of a k_dft_reg workgroup: one per 16 elements of a tile of at most
 // 2^13 elements.
 static constexpr int DFT_REG_MAX_THREADS = 512;
@@ -184,10 +185,10 @@ __global__ void __launch_bounds__(DFT_REG_MAX_THREADS) __attribute__((amdgpu_wav
   // adjacent tiles on one XCD: a line that two tiles' 8 T-byte runs share
   // (column reads, transposed stores) meets in one L2
   const uint64_t seq0 = (uint64_t)xcd_tile(blockIdx.x, gridDim.x) << a.log_T;
-  // one work item per thread: the launch gives the tile's total / 16 threads,
+  // One work item per thread: the launch gives the tile's total / 16 threads,
   // so each thread moves exactly 16 elements in and out; its 16 global loads
-  // are issued together before the LDS stores
-  // element offsets from the tile's first element fit 32 bits (n <= 2^24)
+  // are issued together before the LDS stores.  Element offsets from the
+  // tile's first element fit 32 bits (n <= 2^24).
   const bool in_rows = a.in_elem_stride == 1;
   {
     const uint64_t *in0 = a.in + seq0 * a.in_seq_stride;
