@@ -59,11 +59,15 @@ def test_large_vs_oracle(log_n):
         assert np.array_equal(back, a)
 
 
-@pytest.mark.parametrize("log_n", [3, 10, 13, 17, 23, 24])
-def test_noncanonical_inputs_count_as_residues(log_n):
+@pytest.mark.parametrize("log_n,tiles", [(3, None), (10, None), (13, None), (17, None), (17, (8, 2, 1)),
+                                         (20, None), (21, (11, 1, 2)), (22, None), (23, None), (24, None)])
+def test_noncanonical_inputs_count_as_residues(log_n, tiles):
     """Inputs in [p, 2^64) transform as their residue mod p at every size
-    (the single-pass, LDS and register-resident paths alike)."""
-    _say(f"ntt noncanonical log_n={log_n}")
+    (the single-pass, radix-2 LDS, register-stage LDS and register-resident
+    paths alike; the tile choices put pass 1 on the register-stage kernel)."""
+    _say(f"ntt noncanonical log_n={log_n} tiles={tiles}")
+    if tiles:
+        zntt.set_tiles(log_n, False, *tiles)
     n = 1 << log_n
     rng = np.random.default_rng(500 + log_n)
     a = rng.integers(0, P, size=n, dtype=np.uint64)
@@ -72,9 +76,13 @@ def test_noncanonical_inputs_count_as_residues(log_n):
     big[sel] = (a[sel] % np.uint64(0xFFFFFFFF)) + np.uint64(P)  # p + r, r < 2^32 - 1: still a u64, = r mod p
     red = big.copy()
     red[sel] = big[sel] - np.uint64(P)
-    with zntt.NttGpuDriver(log_n) as d:
-        assert np.array_equal(d.simple_evaluate(big, zntt.FORWARD), oracle.ntt(red))
-        assert np.array_equal(d.simple_evaluate(big, zntt.INVERSE), oracle.ntt(red, inverse=True))
+    try:
+        with zntt.NttGpuDriver(log_n) as d:
+            assert np.array_equal(d.simple_evaluate(big, zntt.FORWARD), oracle.ntt(red))
+            assert np.array_equal(d.simple_evaluate(big, zntt.INVERSE), oracle.ntt(red, inverse=True))
+    finally:
+        if tiles:
+            zntt.set_tiles(log_n, False)
 
 
 def test_edge_values():
