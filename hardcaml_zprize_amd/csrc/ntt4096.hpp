@@ -20,6 +20,7 @@
 #include <utility>
 
 #include "gl64.hpp"
+#include "xcd.hpp"
 
 namespace zp {
 
@@ -235,13 +236,6 @@ struct Ntt4096Args {
   uint64_t *const *out_tab;
 };
 
-// Workgroup id -> tile, so the workgroups dispatched to one XCD (round robin
-// over 8) take adjacent tiles and share their L2 lines.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t g, uint32_t tiles) {
-  const uint32_t per = tiles >> 3;
-  if ((tiles & 7) != 0 || per == 0) return g;
-  return (g & 7) * per + (g >> 3);
-}
 
 // LDS exchange index (u32 words).  The row stride 65 T and the inner
 // strides are chosen so that both the writers (consecutive lanes) and the
