@@ -738,6 +738,19 @@ def measured_traffic(kernel):
     return None
 
 
+def split_label(log_n):
+    """The 4-step split the library uses for one transform of 2^log_n points
+    (built-in tiles): 4096 x 4096 at 2^24."""
+    from hardcaml_zprize_amd import ntt as zntt
+
+    if log_n >= 24:
+        return "4096x4096"
+    if log_n < zntt.TILE_MIN_LOG:  # up to 2^9: one workgroup, one pass
+        return "single-pass"
+    n1 = zntt.get_tiles(log_n, False)[0]
+    return f"{1 << n1}x{1 << (log_n - n1)}"
+
+
 def ntt_roofline(args, ntt):
     """HBM roofline of the 2^24 transform: 16 n algorithmic bytes (one read +
     one write of the vector) over the whole-transform device time.  Each of
@@ -748,7 +761,8 @@ def ntt_roofline(args, ntt):
     achieved = alg / (ntt["ms_per_step"] * 1e-3) / 1e9
     per_kernel = []
     traffic = 0
-    for name, kms in (("k_ntt4096_cols", ntt["ms_pass1"]), ("k_ntt4096r16_rows", ntt["ms_pass2"])):
+    names = ("k_ntt4096_cols", "k_ntt4096r16_rows") if args.log_n == 24 else ("pass 1", "pass 2")
+    for name, kms in zip(names, (ntt["ms_pass1"], ntt["ms_pass2"])):
         a = 8 * n / (kms * 1e-3) / 1e9
         t = measured_traffic(name) if args.log_n == 24 else None
         traffic = None if (t is None or traffic is None) else traffic + t
@@ -800,7 +814,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded uniform in [0,p))",
-        "config": {"workload": f"Goldilocks forward NTT n=2^{args.log_n} in place (4096x4096 4-step), "
+        "config": {"workload": f"Goldilocks forward NTT n=2^{args.log_n} in place ({split_label(args.log_n)} 4-step), "
                                f"resident in HBM, seed {args.seed}", "log_n": args.log_n},
         "roofline": ntt_roofline(args, ntt),
     }
