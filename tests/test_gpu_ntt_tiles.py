@@ -92,3 +92,38 @@ def test_autotuner_ntt_leaves_valid_applied_choices():
     a = _rand(1 << 15, 99)
     with zntt.NttGpuDriver(15) as d:
         assert np.array_equal(d.simple_evaluate(a, zntt.FORWARD), oracle.ntt(a))
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_committed_table_every_size(batched):
+    """The committed autotuner table (what MixedBatch, the server and the bench
+    apply): every size class 2^10 .. 2^23, forward and inverse, equals the
+    oracle; batched runs two jobs of the size on grid.y."""
+    from hardcaml_zprize_amd import autotune
+
+    tiles = autotune.load_ntt_tiles().get("batched" if batched else "single", {})
+    assert sorted(tiles) == list(range(zntt.TILE_MIN_LOG, zntt.TILE_MAX_LOG + 1))
+    autotune.apply_ntt_tiles({"batched" if batched else "single": tiles})
+    for lg in sorted(tiles):
+        assert zntt.get_tiles(lg, batched) == tiles[lg]
+        _say(f"ntt committed tiles log_n={lg} batched={batched}: {tiles[lg]}")
+        xs = [_rand(1 << lg, 900 + 2 * lg + i) for i in range(2)]
+        if not batched:
+            with zntt.NttGpuDriver(lg) as d:
+                assert np.array_equal(d.simple_evaluate(xs[0], zntt.FORWARD), oracle.ntt(xs[0]))
+                assert np.array_equal(d.simple_evaluate(xs[1], zntt.INVERSE), oracle.ntt(xs[1], inverse=True))
+            continue
+        bufs = [zmsm.DeviceBuffer(8 << lg).upload(x) for x in xs]
+        try:
+            b = zntt.NttBatch([x.ptr for x in bufs], [lg, lg], [0, 1])
+            try:
+                b.launch(use_graph=True)
+                call("zp_device_synchronize")
+            finally:
+                b.close()
+            got = [buf.download(np.empty(1 << lg, dtype=np.uint64)) for buf in bufs]
+            assert np.array_equal(got[0], oracle.ntt(xs[0])), lg
+            assert np.array_equal(got[1], oracle.ntt(xs[1], inverse=True)), lg
+        finally:
+            for buf in bufs:
+                buf.free()
