@@ -3,8 +3,9 @@
 // A transform of n = 2^k points:
 //   k <= 9: one workgroup holds the whole vector in LDS (a single pass);
 //   k  > 9: the 4-step of libs/hardcaml_ntt/src/reference_model.ml:92-105 with
-//            n = n1 * n2 (n1 = 2^ceil(k/2)), as the FPGA's two phases
-//            (zprize/ntt/host/driver.cpp:201-231):
+//            n = n1 * n2 (by default n1 = 2^ceil(k/2) up to 2^20 and 4096
+//            from 2^21; the size-class tile table may choose another split),
+//            as the FPGA's two phases (zprize/ntt/host/driver.cpp:201-231):
 //     pass 1  column DFTs of length n1 over the n1 x n2 row-major matrix,
 //             fused twiddle w_n^(j2 * k1), written to a scratch vector;
 //     pass 2  row DFTs of length n2, written transposed into the data vector
